@@ -1,0 +1,217 @@
+"""Benchmark of the soft-decision receive tail on one MI355X.
+
+Workload (one GPU): the rate-4/5 QC-LDPC code drawn from seed 104 (10 x 50
+base matrix, column weights 3-4, shifts in [0, 360), Z = 360, n = 18000,
+k = 14400); 4096 codewords of uniform info bits, encoded, Gray 64-QAM,
+complex64, AWGN at 0.3 dB above the code's measured waterfall
+(commkit_amd.operating).  A step is one pass of the hot path over the batch
+already resident in HBM: max-log float32 LLRs of the 12,288,000 symbols
+(``compute_llr``), then layered normalized min-sum (scale 0.875, at most 50
+iterations, syndrome early stop) of the 4096 frames (``decode``).
+
+Prints one JSON line: decoded info Gb/s over the timed steps, the frame
+latency of a step (every frame of a batch completes with it), post-FEC BER
+and FER of the decoded bits, the roofline of the dominant kernel (the
+decoder: algorithmic bytes per launch over its average duration from HIP
+events on the launch stream) and the CPU baseline (the oracle's C
+restatement of the reference CPU path, timed on a bounded sample of the
+same workload on this host's cores).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+CODE = "r4/5-z360"
+ORDER = 64
+FRAMES = 4096
+MARGIN_DB = 0.3
+
+
+def _parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=FRAMES)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the baseline sample")
+    return ap.parse_args()
+
+
+def decoder_bytes(code, iters: np.ndarray) -> float:
+    """Algorithmic HBM bytes of one decode launch: per frame 4n (LLRs in) + 4n
+    (APP out) + k (bits) + 5 (iterations, valid) + I_f * 8E (R read + write)."""
+    E = code.checks.size
+    per = 8 * code.n + code.k + 5
+    return float(iters.size * per + 8.0 * E * iters.astype(np.float64).sum())
+
+
+def cpu_baseline(code, constellation, rx_host: np.ndarray, info_host: np.ndarray,
+                 noise_var: float, target_s: float) -> dict:
+    """The oracle's C restatement (max-log LLRs + layered min-sum) on a sample."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from commkit_amd.mapping.demap import opposite_levels
+    from commkit_amd.mapping.constellation import pack_bits
+    from oracle import cpu
+
+    re, im = constellation._axes
+    closed = []
+    for a in (re, im):
+        lo, hi = opposite_levels(a.labels)
+        closed.append((a.levels, pack_bits(a.labels), lo, hi, a.bits))
+    inv = float(np.float32(1.0 / noise_var))
+    row_ptr, cols, _ = code._schedule
+    sym_per_frame = code.n // constellation.bits_per_symbol
+
+    def run(frames: int) -> float:
+        t0 = time.perf_counter()
+        x = rx_host[:frames].reshape(-1)
+        llr = cpu.llr_separable_maxlog(x, closed, inv).reshape(frames, code.n)
+        full = np.zeros((frames, code.num_variables), np.float32)
+        full[:, : code.n] = llr
+        cpu.ldpc_layered(full, row_ptr, cols, sum_product=False, scale=0.875,
+                         max_iterations=50, early_stop=True)  # fmt: skip
+        return time.perf_counter() - t0
+
+    probe = min(16, rx_host.shape[0])
+    t = run(probe)
+    frames = int(min(rx_host.shape[0], max(probe, probe * target_s / max(t, 1e-3))))
+    times = [run(frames) for _ in range(1)]
+    dt = float(np.median(times))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    del sym_per_frame
+    return {
+        "value": frames * code.k / dt / 1e9,
+        "unit": "Gb/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{frames} of the {rx_host.shape[0]} frames (max-log LLRs + layered "
+                  f"min-sum, C/OpenMP oracle, {threads} threads), {dt:.2f} s",
+    }
+
+
+def main() -> int:
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(frames shard by whole codewords across GPUs; no multi-GPU run here).",
+              file=sys.stderr)  # fmt: skip
+        return 2
+    import torch
+
+    from commkit_amd import Constellation, MinSum
+    from commkit_amd.coding.ldpc import _decode
+    from commkit_amd.mapping.demap import llr_any
+    from commkit_amd.operating import waterfall_db
+    from commkit_amd.workloads import make_frames, seeded_code
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    code = seeded_code(CODE)
+    c = Constellation.qam(ORDER)
+    esn0 = round(waterfall_db(CODE, ORDER) + MARGIN_DB, 3)
+    F = args.frames
+    fr = make_frames(code, c, esn0, F, seed=0, device=dev)
+    nv = float(fr.noise_var[0])
+    rx = fr.rx.contiguous()
+    method = MinSum(max_iterations=50)
+    torch.cuda.synchronize()
+
+    def step():
+        llr = llr_any(c, rx, nv, "maxlog")
+        return _decode(code, llr.reshape(F, code.n), method)
+
+    for _ in range(args.warmup):
+        out = step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    step_ms = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        ev[s][0].record(stream)
+        llr = llr_any(c, rx, nv, "maxlog")
+        ev[s][1].record(stream)
+        ev[s][2].record(stream)
+        out = _decode(code, llr.reshape(F, code.n), method)
+        ev[s][3].record(stream)
+        ev[s][3].synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    demap_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    decode_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in ev]))
+
+    bits, app, valid, iters = out
+    iters_h = iters.cpu().numpy()
+    errors = (bits != fr.info).sum(dim=1).cpu().numpy()
+    ber = float(errors.sum() / (F * code.k))
+    fer = float((errors > 0).mean())
+    info_bits = F * code.k
+    value = args.steps * info_bits / elapsed / 1e9
+    dec_bytes = decoder_bytes(code, iters_h)
+    achieved = dec_bytes / (decode_ms * 1e-3) / 1e9
+    demap_bytes = rx.numel() * (8 + 4 * c.bits_per_symbol)
+    line = {
+        "metric": "decoded info Gb/s at <=1e-5 post-FEC BER; p50/p99 frame latency",
+        "value": round(value, 3),
+        "unit": "Gb/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded code, uniform info bits, AWGN)",
+        "config": {
+            "workload": "rate-4/5 QC-LDPC n=18000 Z=360 (seed 104), 4096 codewords, Gray "
+                        "64-QAM, max-log LLRs, layered NMS 0.875, <=50 iterations",
+            "frames": F,
+            "esn0_db": esn0,
+            "code": CODE,
+            "constellation": f"{ORDER}-QAM",
+        },
+        "frame_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
+                             "p99": round(float(np.percentile(step_ms, 99)), 3)},
+        "post_fec_ber": ber,
+        "fer": fer,
+        "mean_iterations": float(iters_h.mean()),
+        "kernels_ms": {"demap": round(demap_ms, 4), "decode": round(decode_ms, 4)},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "ldpc_qc_layered (decode)",
+            "bytes_per_launch": dec_bytes,
+        },
+        "demap_roofline": {
+            "achieved": round(demap_bytes / (demap_ms * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "bytes_per_launch": demap_bytes,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(code, c, fr.rx.cpu().numpy(), fr.info.cpu().numpy(),
+                                            nv, args.cpu_seconds)  # fmt: skip
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,621 @@
+"""LDPC codes: :class:`LDPCCode`, :class:`MinSum` and :class:`SumProduct`.
+
+A code is its sparse parity-check matrix ``H`` (``m x N``), optionally with
+the last ``punctured`` bits of each codeword not transmitted, as
+``commkit/coding/ldpc.py:53-125``.  Built from ``H``
+(:meth:`LDPCCode.from_parity_check`) or from a quasi-cyclic base matrix
+(:meth:`LDPCCode.quasi_cyclic`, ``ldpc.py:182-214``: shift ``s`` puts row
+``r``'s one at column ``(r + s) mod Z``); a quasi-cyclic code keeps its base
+matrix, so the decoder computes columns from shifts.
+
+Decoding schedule (``ldpc.py:283-296,422-442``): rows are coloured greedily
+in index order, each taking the first layer none of whose rows shares a
+bit, and stably sorted by colour.  For a quasi-cyclic code every row of a
+block row takes the same colour (each nonzero block is a permutation, so
+earlier block rows sharing a block column use every one of its bits), so a
+layer is a union of whole block rows and the block rows run in colour
+order.
+
+Encoding is systematic, information bits first (``ldpc.py:445-536``):
+a dual-diagonal parity part is accumulated, an invertible one (``m <=
+8192``) is inverted over GF(2), otherwise a dense systematic generator comes
+from row reduction with the rightmost pivots.
+
+Decoding runs ``ck_ldpc_decode`` of ``libck_hip.so``: layered (the
+reference's schedule) or flooding (an addition: every check uses the
+previous iteration's APP, then ``APP = L + sum R`` in ascending check
+order), normalized/offset min-sum or sum-product, syndrome early stop.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import logging
+from dataclasses import dataclass
+from functools import cached_property
+from typing import Any, NamedTuple
+
+import numpy as np
+
+from .. import _native
+from .._array import require_finite, require_int
+from ..backend import device_tables
+from .gf2 import inverse, row_reduce
+
+__all__ = ["LDPCCode", "MinSum", "SumProduct"]
+
+logger = logging.getLogger("commkit_amd")
+
+# Largest parity part inverted densely (m x m bits).
+_MAX_INVERSE = 8192
+# Largest m * N elimination of the dense-generator fallback.
+_MAX_DENSE = 1 << 27
+# Device workspace per decode launch (bytes); larger batches are chunked.
+_GPU_WORKSPACE = 1 << 31
+
+_SCHEDULES = ("layered", "flooding")
+
+
+@dataclass(frozen=True, eq=False)
+class LDPCCode:
+    """LDPC code given by the edges of its sparse parity-check matrix.
+
+    Parameters
+    ----------
+    checks, variables : array_like
+        Row and column of each nonzero entry of ``H``, ``(E,)`` integers,
+        without duplicates.  Every row has at least two entries.
+    num_checks, num_variables : int
+        ``m`` and ``N`` (the mother codeword length).
+    punctured : int, default 0
+        Trailing codeword bits not transmitted (parity bits); ``n = N - punctured``.
+    """
+
+    checks: np.ndarray
+    variables: np.ndarray
+    num_checks: int
+    num_variables: int
+    punctured: int = 0
+
+    def __post_init__(self) -> None:
+        rows = np.asarray(self.checks)
+        cols = np.asarray(self.variables)
+        for name in ("num_checks", "num_variables", "punctured"):
+            require_int(getattr(self, name), name)
+        m, n = int(self.num_checks), int(self.num_variables)
+        if rows.ndim != 1 or rows.shape != cols.shape or rows.size == 0:
+            raise ValueError("checks and variables must be equal-length 1-D arrays.")
+        if rows.dtype.kind not in "iu" or cols.dtype.kind not in "iu":
+            raise TypeError(
+                f"checks and variables must be integers, got {rows.dtype} and {cols.dtype}."
+            )
+        if m < 1 or n <= m:
+            raise ValueError(f"need 1 <= num_checks < num_variables, got {m} and {n}.")
+        if rows.min() < 0 or rows.max() >= m or cols.min() < 0 or cols.max() >= n:
+            raise ValueError("checks or variables out of range.")
+        order = np.lexsort((cols, rows))
+        rows, cols = rows[order].astype(np.int32), cols[order].astype(np.int32)
+        flat = rows.astype(np.int64) * n + cols
+        if np.any(flat[1:] == flat[:-1]):
+            raise ValueError("H has duplicate entries.")
+        if np.bincount(rows, minlength=m).min() < 2:
+            raise ValueError("every parity check needs at least two bits.")
+        p = require_int(self.punctured, "punctured", low=0)
+        rows.setflags(write=False)
+        cols.setflags(write=False)
+        object.__setattr__(self, "checks", rows)
+        object.__setattr__(self, "variables", cols)
+        object.__setattr__(self, "num_checks", m)
+        object.__setattr__(self, "num_variables", n)
+        object.__setattr__(self, "punctured", p)
+        if self._encoder.info.max(initial=-1) >= n - p:
+            raise ValueError("punctured bits must be parity bits.")
+
+    # -- sizes -------------------------------------------------------------------
+
+    @property
+    def n(self) -> int:
+        """Transmitted bits per codeword."""
+        return self.num_variables - self.punctured
+
+    @property
+    def k(self) -> int:
+        """Information bits per codeword."""
+        return int(self._encoder.info.size)
+
+    @property
+    def rate(self) -> float:
+        return float(self.k / self.n)
+
+    def __repr__(self) -> str:
+        return f"LDPCCode(n={self.n}, k={self.k})"
+
+    def __eq__(self, other: object) -> bool:
+        if self is other:
+            return True
+        if not isinstance(other, LDPCCode):
+            return NotImplemented
+        return (
+            self.num_checks == other.num_checks
+            and self.num_variables == other.num_variables
+            and self.punctured == other.punctured
+            and np.array_equal(self.checks, other.checks)
+            and np.array_equal(self.variables, other.variables)
+        )
+
+    def __hash__(self) -> int:
+        h = self.__dict__.get("_hash")
+        if h is None:
+            h = hash((self.num_checks, self.num_variables, self.punctured,
+                      self.checks.tobytes(), self.variables.tobytes()))  # fmt: skip
+            self.__dict__["_hash"] = h
+        return h
+
+    def parity_check(self) -> Any:
+        """``H`` as a ``scipy.sparse.csr_array``, int8, ``(m, N)``."""
+        import scipy.sparse
+
+        return scipy.sparse.csr_array(
+            (np.ones(self.checks.size, np.int8), (self.checks, self.variables)),
+            shape=(self.num_checks, self.num_variables),
+        )
+
+    # -- constructors ------------------------------------------------------------
+
+    @classmethod
+    def from_parity_check(cls, parity_check: Any, *, punctured: int = 0) -> "LDPCCode":
+        """The code of a 0/1 parity-check matrix (dense or scipy.sparse)."""
+        import scipy.sparse
+
+        if not scipy.sparse.issparse(parity_check):
+            parity_check = np.asarray(parity_check)
+        if getattr(parity_check, "ndim", None) != 2:
+            raise ValueError(
+                f"parity_check must be 2-D, got {getattr(parity_check, 'ndim', None)}-D."
+            )
+        h = scipy.sparse.coo_array(parity_check)
+        h.sum_duplicates()
+        h.eliminate_zeros()
+        if np.any(h.data != 1):
+            raise ValueError("parity_check must contain only 0 and 1.")
+        code = cls(h.row, h.col, h.shape[0], h.shape[1], punctured)
+        _warn_if_4_cycles(code)
+        return code
+
+    @classmethod
+    def quasi_cyclic(cls, base: Any, *, lifting: int, punctured: int = 0) -> "LDPCCode":
+        """Quasi-cyclic code from a base matrix of circulant shifts (-1: zero block)."""
+        b = np.asarray(base)
+        z = require_int(lifting, "lifting", low=1)
+        if b.ndim != 2 or b.size == 0 or b.dtype.kind not in "iu":
+            raise ValueError("base must be a non-empty 2-D integer matrix.")
+        if b.min() < -1:
+            raise ValueError("base entries must be shifts >= 0 or -1.")
+        bi, bj = np.nonzero(b >= 0)
+        r = np.arange(z)
+        rows = (bi[:, None] * z + r).ravel()
+        cols = (bj[:, None] * z + (r + b[bi, bj][:, None]) % z).ravel()
+        code = cls(rows, cols, b.shape[0] * z, b.shape[1] * z, punctured)
+        base_ro = (np.where(b >= 0, b % z, -1)).astype(np.int64)
+        base_ro.setflags(write=False)
+        code.__dict__["_qc"] = (base_ro, z)
+        _warn_if_4_cycles(code)
+        return code
+
+    # -- cached structure --------------------------------------------------------
+
+    @cached_property
+    def _layer_of_row(self) -> np.ndarray:
+        """First-fit layer (colour) of each natural row."""
+        m = self.num_checks
+        ptr = np.concatenate([[0], np.cumsum(np.bincount(self.checks, minlength=m))])
+        return _first_fit(ptr, self.variables, self.num_variables)
+
+    @cached_property
+    def _schedule_rows(self) -> np.ndarray:
+        """Natural index of each row in schedule order (rows stably sorted by layer)."""
+        return np.argsort(self._layer_of_row, kind="stable").astype(np.int64)
+
+    @cached_property
+    def _schedule(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """``(row_ptr, cols, layer_ptr)``: CSR of H with rows in layer order."""
+        m = self.num_checks
+        ptr = np.concatenate([[0], np.cumsum(np.bincount(self.checks, minlength=m))])
+        color = self._layer_of_row
+        order = self._schedule_rows
+        degree = np.diff(ptr)[order]
+        row_ptr = np.concatenate([[0], np.cumsum(degree)]).astype(np.int32)
+        cols = np.concatenate([self.variables[ptr[r] : ptr[r + 1]] for r in order])
+        counts = np.bincount(color)
+        layer_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        logger.debug("%r: %d layers.", self, counts.size)
+        return row_ptr, cols.astype(np.int32), layer_ptr
+
+    @cached_property
+    def _encoder(self) -> "_Encoder":
+        return _build_encoder(self)
+
+    @cached_property
+    def _qc_view(self) -> dict[str, np.ndarray] | None:
+        """The quasi-cyclic tables of ``ck_ldpc_code`` (None for a non-QC code).
+
+        Block rows in schedule order; slots of a block row in ascending block
+        column (= ascending column of every row of the block row).
+        """
+        qc = self.__dict__.get("_qc")
+        if qc is None:
+            return None
+        base, z = qc
+        row_ptr, _, _ = self._schedule
+        order = self._schedule_rows
+        block_order = order[::z] // z  # rows of a block row stay together
+        if not np.array_equal(order, (block_order[:, None] * z + np.arange(z)).ravel()):
+            return None  # defensive: schedule not block-aligned
+        mb, nb = base.shape
+        slot_ptr = [0]
+        slot_col, slot_shift, slot_pos, slot_row, slot_nat = [], [], [], [], []
+        for sbr, br in enumerate(block_order):
+            cols = np.flatnonzero(base[br] >= 0)
+            slot_col += cols.tolist()
+            slot_shift += base[br, cols].tolist()
+            slot_pos += list(range(cols.size))
+            slot_row += [sbr] * cols.size
+            slot_nat += [int(br)] * cols.size
+            slot_ptr.append(slot_ptr[-1] + cols.size)
+        slot_col_a = np.asarray(slot_col, np.int32)
+        nat = np.asarray(slot_nat, np.int64)
+        col_slot = np.lexsort((nat, slot_col_a)).astype(np.int32)
+        col_ptr = np.concatenate([[0], np.cumsum(np.bincount(slot_col_a, minlength=nb))])
+        degree = np.diff(np.asarray(slot_ptr))
+        return {
+            "lifting": np.int32(z),
+            "base_rows": np.int32(mb),
+            "base_cols": np.int32(nb),
+            "row_ptr": np.asarray(slot_ptr, np.int32),
+            "slot_col": slot_col_a,
+            "slot_shift": np.asarray(slot_shift, np.int32),
+            "row_id": np.asarray(block_order, np.int32),
+            "col_ptr": col_ptr.astype(np.int32),
+            "col_slot": col_slot,
+            "slot_pos": np.asarray(slot_pos, np.int32),
+            "slot_row": np.asarray(slot_row, np.int32),
+            "max_degree": np.int32(degree.max()),
+        }
+
+    @cached_property
+    def _var_view(self) -> tuple[np.ndarray, np.ndarray]:
+        """``(var_ptr, var_edges)``: per variable, its schedule-CSR edge ids in
+        ascending natural check index (the flooding sum order)."""
+        row_ptr, cols, _ = self._schedule
+        order = self._schedule_rows
+        deg = np.diff(row_ptr)
+        nat_row = np.repeat(order, deg)  # natural check of each schedule edge
+        edges = np.lexsort((nat_row, cols)).astype(np.int32)
+        var_ptr = np.concatenate(
+            [[0], np.cumsum(np.bincount(cols, minlength=self.num_variables))]
+        ).astype(np.int32)
+        return var_ptr, edges
+
+
+class _Encoder(NamedTuple):
+    """How a code encodes: ``kind`` in {accumulate, inverse, dense}, the
+    information and parity positions and the host tables."""
+
+    kind: str
+    info: np.ndarray
+    parity: np.ndarray
+    tables: tuple[np.ndarray, ...]
+
+
+@dataclass(frozen=True)
+class MinSum:
+    """Normalized (and offset) min-sum: ``R = sign * max(scale * min - offset, 0)``.
+
+    ``schedule`` is ``"layered"`` (the reference's, ``commkit/coding/ldpc.py:
+    317-349``) or ``"flooding"`` (an addition).
+    """
+
+    scale: float = 0.875
+    offset: float = 0.0
+    max_iterations: int = 50
+    early_stop: bool = True
+    schedule: str = "layered"
+
+    def __post_init__(self) -> None:
+        if not require_finite(self.scale, "scale", low=0.0) <= 1.0:
+            raise ValueError(f"scale must be in (0, 1], got {self.scale}.")
+        require_finite(self.offset, "offset", low=0.0, inclusive=True)
+        require_int(self.max_iterations, "max_iterations", low=1)
+        _check_schedule(self.schedule)
+
+
+@dataclass(frozen=True)
+class SumProduct:
+    """Sum-product: ``R = sign * phi(sum phi(|Q|))``, ``phi(x) = -log tanh(x/2)``
+    on ``|x|`` clipped to ``[1e-7, 30]`` (``commkit/coding/ldpc.py:351-371``)."""
+
+    max_iterations: int = 50
+    early_stop: bool = True
+    schedule: str = "layered"
+
+    def __post_init__(self) -> None:
+        require_int(self.max_iterations, "max_iterations", low=1)
+        _check_schedule(self.schedule)
+
+
+def _check_schedule(schedule: str) -> None:
+    if schedule not in _SCHEDULES:
+        raise ValueError(f"schedule must be one of {_SCHEDULES}, got {schedule!r}.")
+
+
+# -----------------------------------------------------------------------------
+# Construction
+# -----------------------------------------------------------------------------
+
+
+def _warn_if_4_cycles(code: LDPCCode) -> None:
+    h = code.parity_check().astype(np.int32)
+    overlap = (h @ h.T).tocoo()
+    if np.any((overlap.data > 1) & (overlap.row != overlap.col)):
+        logger.warning("%r has 4-cycles; belief propagation degrades on them.", code)
+
+
+def _first_fit(ptr: np.ndarray, variables: np.ndarray, n: int) -> np.ndarray:
+    """Layer of each row: the first layer none of whose earlier rows shares a bit.
+
+    Each bit keeps the set of layers using it as a Python-integer mask.
+    """
+    masks = [0] * n
+    cols = variables.tolist()
+    bounds = ptr.tolist()
+    color = np.empty(len(bounds) - 1, dtype=np.int64)
+    for r in range(len(bounds) - 1):
+        row = cols[bounds[r] : bounds[r + 1]]
+        used = 0
+        for v in row:
+            used |= masks[v]
+        free = ~used & (used + 1)  # lowest clear bit
+        c = free.bit_length() - 1
+        for v in row:
+            masks[v] |= free
+        color[r] = c
+    return color
+
+
+def _build_encoder(code: LDPCCode) -> _Encoder:
+    m, n = code.num_checks, code.num_variables
+    rows, cols = code.checks, code.variables
+    k = n - m
+    in_parity = cols >= k
+    p_rows, p_cols = rows[in_parity], cols[in_parity] - k
+    info_rows, info_cols = rows[~in_parity], cols[~in_parity]
+    expected = np.sort(
+        np.concatenate([np.arange(m) * (m + 1), np.arange(m - 1) * (m + 1) + m])
+    )
+    if p_rows.size == 2 * m - 1 and np.array_equal(
+        np.sort(p_rows.astype(np.int64) * m + p_cols), expected
+    ):
+        return _Encoder("accumulate", np.arange(k), np.arange(k, n), (info_rows, info_cols))
+    if m <= _MAX_INVERSE:
+        hp = np.zeros((m, m), dtype=np.int8)
+        hp[p_rows, p_cols] = 1
+        try:
+            inv = inverse(hp)
+        except ValueError:
+            pass
+        else:
+            return _Encoder("inverse", np.arange(k), np.arange(k, n), (info_rows, info_cols, inv))
+    if m * n > _MAX_DENSE:
+        raise ValueError(
+            f"LDPCCode ({m} x {n}): the parity part is neither dual-diagonal nor "
+            "invertible, and H is too large for a dense systematic generator."
+        )
+    h = np.zeros((m, n), dtype=np.int8)
+    h[rows, cols] = 1
+    rref, pivots = row_reduce(h, reverse=True)
+    free = np.setdiff1d(np.arange(n), pivots)
+    # c[pivots[i]] = sum over free f of rref[i, f] c[f]
+    return _Encoder("dense", free, pivots, (rref[:, free].T.astype(np.int8),))
+
+
+# -----------------------------------------------------------------------------
+# Encoding
+# -----------------------------------------------------------------------------
+
+
+def _encode(code: LDPCCode, bits: Any) -> Any:
+    """Encode ``(R, k)`` int8 device rows into ``(R, n)`` int8 codewords."""
+    import torch
+
+    enc = code._encoder
+    m, k, N = code.num_checks, code.k, code.num_variables
+    dev = bits.device
+    u = bits.to(torch.float32)
+    if enc.kind == "dense":
+        (g,) = device_tables(code, "enc_dense", dev, lambda: (enc.tables[0].astype(np.float32),))
+        parity = (u @ g).to(torch.int64) & 1
+    else:
+        info_rows, info_cols = enc.tables[0], enc.tables[1]
+
+        def build_hi() -> tuple[np.ndarray]:
+            hi = np.zeros((m, k), dtype=np.float32)
+            hi[info_rows, info_cols] = 1.0
+            return (hi,)
+
+        (hi,) = device_tables(code, "enc_hi", dev, build_hi)
+        s = (u @ hi.T).to(torch.int64) & 1  # (R, m): syndrome of the info part
+        if enc.kind == "accumulate":
+            parity = torch.cumsum(s, dim=1) & 1
+        else:
+            (inv_t,) = device_tables(
+                code, "enc_inv", dev, lambda: (enc.tables[2].T.astype(np.float32),)
+            )
+            parity = (s.to(torch.float32) @ inv_t).to(torch.int64) & 1
+    words = torch.empty((bits.shape[0], N), dtype=torch.int8, device=dev)
+    if enc.kind == "dense":
+        info_t, par_t = device_tables(code, "enc_pos", dev, lambda: (enc.info, enc.parity))
+        words[:, info_t] = bits.to(torch.int8)
+        words[:, par_t] = parity.to(torch.int8)
+    else:
+        words[:, :k] = bits.to(torch.int8)
+        words[:, k:] = parity.to(torch.int8)
+    return words[:, : code.n]
+
+
+# -----------------------------------------------------------------------------
+# Decoding
+# -----------------------------------------------------------------------------
+
+
+class CkLdpcCode(ctypes.Structure):
+    """``ck_ldpc_code`` of ``include/ck_tail.h``."""
+
+    _fields_ = [
+        ("num_checks", ctypes.c_int32),
+        ("num_variables", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("k", ctypes.c_int32),
+        ("num_edges", ctypes.c_int32),
+        ("num_layers", ctypes.c_int32),
+        ("max_row_degree", ctypes.c_int32),
+        ("row_ptr", ctypes.c_void_p),
+        ("cols", ctypes.c_void_p),
+        ("layer_ptr", ctypes.c_void_p),
+        ("var_ptr", ctypes.c_void_p),
+        ("var_edges", ctypes.c_void_p),
+        ("info", ctypes.c_void_p),
+        ("lifting", ctypes.c_int32),
+        ("base_rows", ctypes.c_int32),
+        ("base_cols", ctypes.c_int32),
+        ("num_slots", ctypes.c_int32),
+        ("qc_row_ptr", ctypes.c_void_p),
+        ("qc_slot_col", ctypes.c_void_p),
+        ("qc_slot_shift", ctypes.c_void_p),
+        ("qc_row_id", ctypes.c_void_p),
+        ("qc_col_ptr", ctypes.c_void_p),
+        ("qc_col_slot", ctypes.c_void_p),
+        ("qc_slot_pos", ctypes.c_void_p),
+        ("qc_slot_row", ctypes.c_void_p),
+    ]
+
+
+class CkLdpcMethod(ctypes.Structure):
+    """``ck_ldpc_method`` of ``include/ck_tail.h``."""
+
+    _fields_ = [
+        ("rule", ctypes.c_int32),
+        ("schedule", ctypes.c_int32),
+        ("scale", ctypes.c_float),
+        ("offset", ctypes.c_float),
+        ("max_iterations", ctypes.c_int32),
+        ("early_stop", ctypes.c_int32),
+        ("kernel", ctypes.c_int32),
+        ("frames_per_group", ctypes.c_int32),
+        ("lanes_per_row", ctypes.c_int32),
+        ("state_in_lds", ctypes.c_int32),
+    ]
+
+
+class DeviceCode(NamedTuple):
+    """A code's ``ck_ldpc_code`` on one device and the tensors it points into."""
+
+    struct: CkLdpcCode
+    keep: tuple
+
+
+def device_code(code: LDPCCode, device: Any) -> DeviceCode:
+    """Upload a code's decoder tables once per device (cached on the code)."""
+    row_ptr, cols, layer_ptr = code._schedule
+    qc = code._qc_view
+    var_ptr, var_edges = code._var_view
+    info = code._encoder.info
+    contiguous_info = info.size == 0 or info.size == info[-1] + 1
+
+    def build() -> tuple[np.ndarray, ...]:
+        base = [row_ptr, cols, layer_ptr, var_ptr, var_edges, info.astype(np.int32)]
+        if qc is not None:
+            base += [qc[key] for key in ("row_ptr", "slot_col", "slot_shift", "row_id",
+                                         "col_ptr", "col_slot", "slot_pos", "slot_row")]  # fmt: skip
+        return tuple(np.ascontiguousarray(a, dtype=np.int32) for a in base)
+
+    t = device_tables(code, "decoder", device, build)
+    deg = np.diff(row_ptr)
+    s = CkLdpcCode()
+    s.num_checks = code.num_checks
+    s.num_variables = code.num_variables
+    s.n = code.n
+    s.k = code.k
+    s.num_edges = int(cols.size)
+    s.num_layers = int(layer_ptr.size - 1)
+    s.max_row_degree = int(deg.max())
+    s.row_ptr, s.cols, s.layer_ptr = t[0].data_ptr(), t[1].data_ptr(), t[2].data_ptr()
+    s.var_ptr, s.var_edges = t[3].data_ptr(), t[4].data_ptr()
+    s.info = None if contiguous_info else t[5].data_ptr()
+    if qc is not None:
+        s.lifting = int(qc["lifting"])
+        s.base_rows = int(qc["base_rows"])
+        s.base_cols = int(qc["base_cols"])
+        s.num_slots = int(qc["slot_col"].size)
+        (s.qc_row_ptr, s.qc_slot_col, s.qc_slot_shift, s.qc_row_id, s.qc_col_ptr,
+         s.qc_col_slot, s.qc_slot_pos, s.qc_slot_row) = (x.data_ptr() for x in t[6:14])  # fmt: skip
+    return DeviceCode(s, t)
+
+
+#: Tile choices per (n, lifting, rule, schedule) written by the autotuner
+#: (commkit_amd.tune); absent keys use the library's defaults.
+TUNED: dict[tuple[int, int, int, int], tuple[int, int, int, int]] = {}
+
+
+def method_struct(code: LDPCCode, method: MinSum | SumProduct) -> CkLdpcMethod:
+    sp = isinstance(method, SumProduct)
+    m = CkLdpcMethod()
+    m.rule = 1 if sp else 0
+    m.schedule = _SCHEDULES.index(method.schedule)
+    m.scale = 1.0 if sp else float(np.float32(method.scale))  # type: ignore[union-attr]
+    m.offset = 0.0 if sp else float(np.float32(method.offset))  # type: ignore[union-attr]
+    m.max_iterations = int(method.max_iterations)
+    m.early_stop = int(bool(method.early_stop))
+    qc = code._qc_view
+    key = (code.num_variables, int(qc["lifting"]) if qc is not None else 0, m.rule, m.schedule)
+    kernel, frames, lanes, state = TUNED.get(key, (0, 0, 0, -1))
+    m.kernel, m.frames_per_group, m.lanes_per_row, m.state_in_lds = kernel, frames, lanes, state
+    return m
+
+
+def _decode(code: LDPCCode, llrs: Any, method: MinSum | SumProduct, *, tiles: Any = None):
+    """Decode ``(R, n)`` float32 device rows: ``(bits, app, valid, iterations)``."""
+    import torch
+
+    dev = llrs.device
+    R = int(llrs.shape[0])
+    llrs = llrs.contiguous()
+    app = torch.empty((R, code.n), dtype=torch.float32, device=dev)
+    bits = torch.empty((R, code.k), dtype=torch.int8, device=dev)
+    iters = torch.empty(R, dtype=torch.int32, device=dev)
+    valid = torch.empty(R, dtype=torch.uint8, device=dev)
+    if R == 0:
+        return bits, app, valid.bool(), iters
+    lib = _native.lib()
+    with torch.cuda.device(dev):
+        dc = device_code(code, dev)
+        ms = method_struct(code, method)
+        if tiles is not None:
+            ms.kernel, ms.frames_per_group, ms.lanes_per_row, ms.state_in_lds = tiles
+        per = lib.ck_ldpc_workspace_bytes(ctypes.byref(dc.struct), ctypes.byref(ms), 1)
+        if per < 0:
+            _native.check(lib.ck_ldpc_decode(ctypes.byref(dc.struct), ctypes.byref(ms),
+                                             None, None, None, None, None, 0, None, None))  # fmt: skip
+            raise RuntimeError("ck_ldpc_workspace_bytes failed")
+        chunk = R if per == 0 else max(1, min(R, _GPU_WORKSPACE // max(per, 1)))
+        ws_bytes = lib.ck_ldpc_workspace_bytes(ctypes.byref(dc.struct), ctypes.byref(ms), chunk)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        stream = _native.stream_of(dev)
+        for r0 in range(0, R, chunk):
+            rc = min(chunk, R - r0)
+            _native.check(lib.ck_ldpc_decode(
+                ctypes.byref(dc.struct), ctypes.byref(ms),
+                llrs.data_ptr() + 4 * r0 * code.n, app.data_ptr() + 4 * r0 * code.n,
+                bits.data_ptr() + r0 * code.k, iters.data_ptr() + 4 * r0,
+                valid.data_ptr() + r0, rc, ws.data_ptr() if ws_bytes else None, stream,
+            ))  # fmt: skip
+    return bits, app, valid.bool(), iters

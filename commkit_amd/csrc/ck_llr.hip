@@ -1,0 +1,453 @@
+// Bit-LLR demapping on gfx950: three streaming kernels, one symbol per lane.
+//
+//   separable_closed  max-log LLRs of a uniform separable lattice: per axis,
+//                     the snapped level against the nearer of the two levels
+//                     just outside its run of equal bits (1 + 2 kb metrics);
+//   separable_levels  the metrics of each axis's levels with the log prior,
+//                     max-log or exact log-sum-exp;
+//   points            the metrics of all M points (any constellation).
+//
+// Semantics follow commkit/mapping/_numba_llr.py:35-235 and
+// commkit/_cuda/src/llr.cu:100-337 (same routing, same exact-LSE floor of
+// 1e-26 below which a half is summed relative to its own maximum).  The
+// arithmetic is this library's own, chosen for accuracy against an FP64
+// evaluation of the definition:
+//
+//   - a difference of two metrics m_a - m_c is never formed by subtracting
+//     the metrics; it is evaluated in factored form,
+//       m_a - m_c = (l_a - l_c) * ((x - l_a) + (x - l_c)) * inv_s2
+//                   + (logp_a - logp_c),
+//     so an LLR near zero (x near a decision boundary) keeps its relative
+//     accuracy instead of cancelling two large metrics;
+//   - the exact sums take exp(m - m_peak) with that factored difference, one
+//     exponential per level (or point), and ocml's expf/logf.
+//
+// Layout: per axis, the tables (levels, log prior, labels, opposite levels,
+// bit positions) are staged in LDS once per workgroup.  Each lane writes its
+// k LLRs into an LDS tile of kThreads rows of stride (k | 1) floats (odd, so
+// lane t's row starts in its own bank) and the workgroup stores the tile to
+// HBM contiguously, as 16-byte vectors where the tile is whole.
+#include "ck_common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr float kLogFloor = -59.867214f;  // logf(1e-26f)
+
+__device__ __forceinline__ int tile_stride(int k) { return k | 1; }
+
+// Stores `count` staged symbols of k LLRs from the tile to out[base * k ...].
+__device__ __forceinline__ void store_tile(const float* tile, float* __restrict__ out,
+                                           long long base, int count, int k) {
+    const int stride = tile_stride(k);
+    float* dst = out + base * k;
+    const int total = count * k;
+    if (count == kThreads) {  // whole tile: 256 * k floats, 16-byte aligned
+        float4* dst4 = reinterpret_cast<float4*>(dst);
+        for (int q = threadIdx.x; q < total / 4; q += kThreads) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = 4 * q + u;
+                const int sym = j / k;
+                v[u] = tile[sym * stride + (j - sym * k)];
+            }
+            dst4[q] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    } else {
+        for (int j = threadIdx.x; j < total; j += kThreads) {
+            const int sym = j / k;
+            dst[j] = tile[sym * stride + (j - sym * k)];
+        }
+    }
+}
+
+// LDS copy of one axis with 2^KB levels.
+template <int KB>
+struct AxisTab {
+    static constexpr int L = 1 << KB;
+    float lev[L];
+    float logp[L];
+    int lab[L];
+    short lo[KB][L];
+    short hi[KB][L];
+    int pos[KB];
+};
+
+template <int KB>
+__device__ void stage_axis(AxisTab<KB>& t, const ck_axis& a, bool with_opposite) {
+    constexpr int L = 1 << KB;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        t.lev[i] = a.levels[i];
+        t.logp[i] = a.log_prior[i];
+        t.lab[i] = a.labels[i];
+    }
+    if (with_opposite) {
+        for (int i = threadIdx.x; i < KB * L; i += blockDim.x) {
+            t.lo[i / L][i % L] = static_cast<short>(a.lo[i]);
+            t.hi[i / L][i % L] = static_cast<short>(a.hi[i]);
+        }
+    }
+    if (threadIdx.x < KB) t.pos[threadIdx.x] = a.pos[threadIdx.x];
+}
+
+// m_a - m_c of two levels of one axis, factored (see the file comment).
+template <int KB>
+__device__ __forceinline__ float level_diff(const AxisTab<KB>& t, float x, float inv_s2,
+                                            int a, int c) {
+    const float la = t.lev[a];
+    const float lc = t.lev[c];
+    return (la - lc) * ((x - la) + (x - lc)) * inv_s2 + (t.logp[a] - t.logp[c]);
+}
+
+// Closed-form max-log LLRs of one axis into row[pos[b]].
+template <int KB>
+__device__ __forceinline__ void axis_closed(const AxisTab<KB>& t, float x, float inv_s2,
+                                            float inv_step, float* row) {
+    constexpr int L = 1 << KB;
+    int s = __float2int_rn((x - t.lev[0]) * inv_step);
+    s = min(max(s, 0), L - 1);
+    float d = x - t.lev[s];
+    // The rounded quotient can land one level off near a midpoint: keep the
+    // nearer of s and its neighbour towards x.
+    const int s2 = d > 0.0f ? s + 1 : s - 1;
+    if (s2 >= 0 && s2 < L) {
+        const float d2 = x - t.lev[s2];
+        if (d2 * d2 < d * d) {
+            s = s2;
+            d = d2;
+        }
+    }
+    const float ls = t.lev[s];
+    const int lab = t.lab[s];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        const int jl = t.lo[b][s];
+        const int jh = t.hi[b][s];
+        int j;
+        if (jl < 0) {
+            j = jh;
+        } else if (jh < 0) {
+            j = jl;
+        } else {
+            const float dl = x - t.lev[jl];
+            const float dh = x - t.lev[jh];
+            j = (dh * dh < dl * dl) ? jh : jl;
+        }
+        j = j < 0 ? s : j;  // a bit constant over the axis: no opposite level
+        const float lo = t.lev[j];
+        // m_own - m_other, the LLR when the snapped level's bit is 0.
+        const float diff = (ls - lo) * (d + (x - lo)) * inv_s2;
+        row[t.pos[b]] = ((lab >> (KB - 1 - b)) & 1) ? -diff : diff;
+    }
+}
+
+// LLRs of one axis from the metrics of its levels into row[pos[b]].
+template <int KB>
+__device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float inv_s2,
+                                            bool exact, float* row) {
+    constexpr int L = 1 << KB;
+    float p0[KB], p1[KB];
+    int a0[KB], a1[KB];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        p0[b] = -ck::kInf;
+        p1[b] = -ck::kInf;
+        a0[b] = 0;
+        a1[b] = 0;
+    }
+#pragma unroll 4
+    for (int l = 0; l < L; ++l) {
+        const float dx = x - t.lev[l];
+        const float m = -(dx * dx) * inv_s2 + t.logp[l];
+        const int lab = t.lab[l];
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            if ((lab >> (KB - 1 - b)) & 1) {
+                if (m > p1[b]) { p1[b] = m; a1[b] = l; }
+            } else {
+                if (m > p0[b]) { p0[b] = m; a0[b] = l; }
+            }
+        }
+    }
+    if (!exact) {
+#pragma unroll
+        for (int b = 0; b < KB; ++b) row[t.pos[b]] = level_diff(t, x, inv_s2, a0[b], a1[b]);
+        return;
+    }
+    // Every level lies in one half of bit 0: the better of its two maxima is
+    // the peak.  A half whose own maximum lies below the peak by more than
+    // -log(1e-26) is summed relative to that maximum instead.
+    const int pk = (p0[0] >= p1[0]) ? a0[0] : a1[0];
+    float w0[KB], w1[KB], s0[KB], s1[KB];
+    unsigned weak0 = 0u, weak1 = 0u;
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        w0[b] = level_diff(t, x, inv_s2, a0[b], pk);
+        w1[b] = level_diff(t, x, inv_s2, a1[b], pk);
+        weak0 |= (w0[b] < kLogFloor ? 1u : 0u) << b;
+        weak1 |= (w1[b] < kLogFloor ? 1u : 0u) << b;
+        s0[b] = 0.0f;
+        s1[b] = 0.0f;
+    }
+#pragma unroll 4
+    for (int l = 0; l < L; ++l) {
+        const float e = expf(level_diff(t, x, inv_s2, l, pk));
+        const int lab = t.lab[l];
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            if ((lab >> (KB - 1 - b)) & 1) {
+                s1[b] += ((weak1 >> b) & 1u) ? expf(level_diff(t, x, inv_s2, l, a1[b])) : e;
+            } else {
+                s0[b] += ((weak0 >> b) & 1u) ? expf(level_diff(t, x, inv_s2, l, a0[b])) : e;
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        const float l0 = logf(s0[b]) + (((weak0 >> b) & 1u) ? w0[b] : 0.0f);
+        const float l1 = logf(s1[b]) + (((weak1 >> b) & 1u) ? w1[b] : 0.0f);
+        row[t.pos[b]] = l0 - l1;
+    }
+}
+
+// CLOSED: closed form (max-log, uniform); else level metrics.
+template <int KB, bool CLOSED>
+__global__ __launch_bounds__(kThreads) void llr_separable_kernel(
+    const float2* __restrict__ x, long long n, ck_axis re, ck_axis im, float inv_s2,
+    int exact, float* __restrict__ out) {
+    constexpr int K = 2 * KB;
+    __shared__ AxisTab<KB> tab[2];
+    __shared__ float tile[kThreads * (K | 1)];
+    stage_axis<KB>(tab[0], re, CLOSED);
+    stage_axis<KB>(tab[1], im, CLOSED);
+    __syncthreads();
+    const float inv_step0 = 1.0f / (tab[0].lev[1] - tab[0].lev[0]);
+    const float inv_step1 = 1.0f / (tab[1].lev[1] - tab[1].lev[0]);
+    float* row = tile + threadIdx.x * tile_stride(K);
+    for (long long base = static_cast<long long>(blockIdx.x) * kThreads; base < n;
+         base += static_cast<long long>(gridDim.x) * kThreads) {
+        const long long i = base + threadIdx.x;
+        if (i < n) {
+            const float2 v = x[i];
+            if (CLOSED) {
+                axis_closed<KB>(tab[0], v.x, inv_s2, inv_step0, row);
+                axis_closed<KB>(tab[1], v.y, inv_s2, inv_step1, row);
+            } else {
+                axis_levels<KB>(tab[0], v.x, inv_s2, exact != 0, row);
+                axis_levels<KB>(tab[1], v.y, inv_s2, exact != 0, row);
+            }
+        }
+        __syncthreads();
+        const long long left = n - base;
+        store_tile(tile, out, base, left < kThreads ? static_cast<int>(left) : kThreads, K);
+        __syncthreads();
+    }
+}
+
+// m_a - m_c of two points, factored per component.
+__device__ __forceinline__ float point_diff(const float2* pts, const float* logp, float xr,
+                                            float xi, float inv_s2, int a, int c) {
+    const float2 pa = pts[a];
+    const float2 pc = pts[c];
+    const float dr = (pa.x - pc.x) * ((xr - pa.x) + (xr - pc.x));
+    const float di = (pa.y - pc.y) * ((xi - pa.y) + (xi - pc.y));
+    return (dr + di) * inv_s2 + (logp[a] - logp[c]);
+}
+
+constexpr int kPointsInLds = 2048;
+
+template <int K>
+__global__ __launch_bounds__(kThreads) void llr_points_kernel(
+    const float2* __restrict__ x, long long n, const float2* __restrict__ g_pts,
+    const float* __restrict__ g_logp, const int* __restrict__ g_lab, int M, float inv_s2,
+    int exact, float* __restrict__ out) {
+    __shared__ float tile[kThreads * (K | 1)];
+    extern __shared__ char dyn[];
+    const bool staged = M <= kPointsInLds;
+    const float2* pts = g_pts;
+    const float* logp = g_logp;
+    const int* lab_tab = g_lab;
+    if (staged) {
+        float2* s_pts = reinterpret_cast<float2*>(dyn);
+        float* s_logp = reinterpret_cast<float*>(s_pts + M);
+        int* s_lab = reinterpret_cast<int*>(s_logp + M);
+        for (int i = threadIdx.x; i < M; i += kThreads) {
+            s_pts[i] = g_pts[i];
+            s_logp[i] = g_logp[i];
+            s_lab[i] = g_lab[i];
+        }
+        pts = s_pts;
+        logp = s_logp;
+        lab_tab = s_lab;
+    }
+    __syncthreads();
+    float* row = tile + threadIdx.x * tile_stride(K);
+    for (long long base = static_cast<long long>(blockIdx.x) * kThreads; base < n;
+         base += static_cast<long long>(gridDim.x) * kThreads) {
+        const long long i = base + threadIdx.x;
+        if (i < n) {
+            const float2 v = x[i];
+            float p0[K], p1[K];
+            int a0[K], a1[K];
+#pragma unroll
+            for (int b = 0; b < K; ++b) {
+                p0[b] = -ck::kInf;
+                p1[b] = -ck::kInf;
+                a0[b] = 0;
+                a1[b] = 0;
+            }
+            for (int m = 0; m < M; ++m) {
+                const float2 p = pts[m];
+                const float dr = v.x - p.x;
+                const float di = v.y - p.y;
+                const float met = -(dr * dr + di * di) * inv_s2 + logp[m];
+                const int lab = lab_tab[m];
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    if ((lab >> (K - 1 - b)) & 1) {
+                        if (met > p1[b]) { p1[b] = met; a1[b] = m; }
+                    } else {
+                        if (met > p0[b]) { p0[b] = met; a0[b] = m; }
+                    }
+                }
+            }
+            if (!exact) {
+#pragma unroll
+                for (int b = 0; b < K; ++b)
+                    row[b] = point_diff(pts, logp, v.x, v.y, inv_s2, a0[b], a1[b]);
+            } else {
+                const int pk = (p0[0] >= p1[0]) ? a0[0] : a1[0];
+                float w0[K], w1[K], s0[K], s1[K];
+                unsigned weak0 = 0u, weak1 = 0u;
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    w0[b] = point_diff(pts, logp, v.x, v.y, inv_s2, a0[b], pk);
+                    w1[b] = point_diff(pts, logp, v.x, v.y, inv_s2, a1[b], pk);
+                    weak0 |= (w0[b] < kLogFloor ? 1u : 0u) << b;
+                    weak1 |= (w1[b] < kLogFloor ? 1u : 0u) << b;
+                    s0[b] = 0.0f;
+                    s1[b] = 0.0f;
+                }
+                for (int m = 0; m < M; ++m) {
+                    const float e = expf(point_diff(pts, logp, v.x, v.y, inv_s2, m, pk));
+                    const int lab = lab_tab[m];
+#pragma unroll
+                    for (int b = 0; b < K; ++b) {
+                        if ((lab >> (K - 1 - b)) & 1) {
+                            s1[b] += ((weak1 >> b) & 1u)
+                                         ? expf(point_diff(pts, logp, v.x, v.y, inv_s2, m, a1[b]))
+                                         : e;
+                        } else {
+                            s0[b] += ((weak0 >> b) & 1u)
+                                         ? expf(point_diff(pts, logp, v.x, v.y, inv_s2, m, a0[b]))
+                                         : e;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    const float l0 = logf(s0[b]) + (((weak0 >> b) & 1u) ? w0[b] : 0.0f);
+                    const float l1 = logf(s1[b]) + (((weak1 >> b) & 1u) ? w1[b] : 0.0f);
+                    row[b] = l0 - l1;
+                }
+            }
+        }
+        __syncthreads();
+        const long long left = n - base;
+        store_tile(tile, out, base, left < kThreads ? static_cast<int>(left) : kThreads, K);
+        __syncthreads();
+    }
+}
+
+int grid_for(long long n) {
+    ck::DeviceInfo info{};
+    if (ck::device_info(&info) != CK_OK) info.num_cus = 256;
+    const long long tiles = (n + kThreads - 1) / kThreads;
+    const long long cap = static_cast<long long>(info.num_cus) * 16;
+    return static_cast<int>(tiles < cap ? tiles : cap);
+}
+
+int32_t check_axis(const ck_axis* a, const char* name, bool closed) {
+    CK_REQUIRE(a != nullptr, "%s axis is NULL", name);
+    CK_REQUIRE(a->num_bits >= 1 && a->num_bits <= 8, "%s axis: num_bits %d not in [1, 8]", name,
+               a->num_bits);
+    CK_REQUIRE(a->num_levels == (1 << a->num_bits), "%s axis: num_levels %d != 2^num_bits", name,
+               a->num_levels);
+    CK_REQUIRE(a->levels && a->log_prior && a->labels && a->pos, "%s axis: NULL table", name);
+    CK_REQUIRE(!closed || (a->lo && a->hi), "%s axis: NULL opposite-level table", name);
+    return CK_OK;
+}
+
+template <bool CLOSED>
+int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_axis* im,
+                         float inv_s2, int exact, float* out, void* stream) {
+    CK_REQUIRE(n >= 0, "n = %lld < 0", static_cast<long long>(n));
+    if (int32_t st = check_axis(re, "re", CLOSED)) return st;
+    if (int32_t st = check_axis(im, "im", CLOSED)) return st;
+    CK_REQUIRE(re->num_bits == im->num_bits,
+               "separable axes must carry the same number of bits (%d, %d)", re->num_bits,
+               im->num_bits);
+    CK_REQUIRE(inv_s2 > 0.0f && inv_s2 < ck::kInf, "inv_s2 must be positive and finite");
+    if (n == 0) return CK_OK;
+    CK_REQUIRE(x != nullptr && out != nullptr, "NULL symbols or output");
+    const int grid = grid_for(n);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const float2* xs = static_cast<const float2*>(x);
+    switch (re->num_bits) {
+#define CK_CASE(KB)                                                                           \
+    case KB:                                                                                  \
+        hipLaunchKernelGGL((llr_separable_kernel<KB, CLOSED>), dim3(grid), dim3(kThreads), 0, \
+                           s, xs, static_cast<long long>(n), *re, *im, inv_s2, exact, out);   \
+        break;
+        CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
+#undef CK_CASE
+    }
+    return ck::launched(CLOSED ? "llr_separable_maxlog" : "llr_separable");
+}
+
+}  // namespace
+
+extern "C" int32_t ck_llr_separable_maxlog(const void* x, int64_t n, const ck_axis* re,
+                                           const ck_axis* im, float inv_s2, float* out,
+                                           void* stream) {
+    return launch_separable<true>(x, n, re, im, inv_s2, 0, out, stream);
+}
+
+extern "C" int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
+                                    const ck_axis* im, float inv_s2, int32_t exact, float* out,
+                                    void* stream) {
+    CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
+    return launch_separable<false>(x, n, re, im, inv_s2, exact, out, stream);
+}
+
+extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,
+                                 const float* log_prior, const int32_t* labels, int32_t M,
+                                 int32_t k, float inv_s2, int32_t exact, float* out,
+                                 void* stream) {
+    CK_REQUIRE(n >= 0, "n = %lld < 0", static_cast<long long>(n));
+    CK_REQUIRE(k >= 1 && k <= 16, "k = %d not in [1, 16]", k);
+    CK_REQUIRE(M == (1 << k), "M = %d != 2^k", M);
+    CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
+    CK_REQUIRE(inv_s2 > 0.0f && inv_s2 < ck::kInf, "inv_s2 must be positive and finite");
+    if (n == 0) return CK_OK;
+    CK_REQUIRE(x && out && points && log_prior && labels, "NULL pointer argument");
+    const int grid = grid_for(n);
+    const size_t dyn = M <= kPointsInLds ? static_cast<size_t>(M) * 16 : 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const float2* xs = static_cast<const float2*>(x);
+    const float2* pts = static_cast<const float2*>(points);
+    switch (k) {
+#define CK_CASE(K)                                                                           \
+    case K:                                                                                  \
+        hipLaunchKernelGGL((llr_points_kernel<K>), dim3(grid), dim3(kThreads), dyn, s, xs,   \
+                           static_cast<long long>(n), pts, log_prior, labels, M, inv_s2,     \
+                           exact, out);                                                      \
+        break;
+        CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
+        CK_CASE(9) CK_CASE(10) CK_CASE(11) CK_CASE(12) CK_CASE(13) CK_CASE(14) CK_CASE(15)
+        CK_CASE(16)
+#undef CK_CASE
+    }
+    return ck::launched("llr_points");
+}

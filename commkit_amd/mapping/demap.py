@@ -1,0 +1,201 @@
+"""Array core of soft demapping: tables, routing and the HIP launches.
+
+Three kernels in ``libck_hip.so`` compute float32 bit LLRs, one symbol per
+lane, with the tables staged in LDS:
+
+- ``ck_llr_separable_maxlog``: max-log LLRs of a uniform separable
+  lattice in closed form, per axis the snapped level against the nearer of
+  the two levels just outside its run of equal bits
+  (``commkit/mapping/_demap.py:177-212``, ``commkit/_cuda/src/llr.cu:262-337``);
+- ``ck_llr_separable``: the metrics of each axis's levels, max-log or
+  exact log-sum-exp, with the per-axis log prior (``_demap.py:326-447``,
+  ``llr.cu:225-259``);
+- ``ck_llr_points``: the metrics of all M points (``_demap.py:245-314``,
+  ``llr.cu:200-223``).
+
+Routing follows ``Constellation.llr`` (``commkit/mapping/constellation.py:
+328-332``) and ``_closed_form`` (``_demap.py:207-212``): separable and
+max-log without a prior takes the closed form, separable otherwise the level
+metrics, anything else the point metrics.  Sign convention: a positive LLR
+means bit 0 is more likely.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import TYPE_CHECKING, Any
+
+import numpy as np
+
+from .. import _native
+from ..backend import device_tables
+from .constellation import pack_bits
+from .lattice import Axis
+
+if TYPE_CHECKING:
+    from .constellation import Constellation
+
+__all__ = ["LLR_MAX_K", "llr_any", "noise_variance", "opposite_levels", "log_prior"]
+
+#: Most bits per symbol the kernels take (``commkit/_cuda/llr.py:12``).
+LLR_MAX_K = 16
+
+
+def noise_variance(
+    noise_var: float | None,
+    esn0_db: float | None,
+    constellation: "Constellation",
+    real: bool,
+    function_name: str,
+) -> float:
+    """Noise variance from ``noise_var`` or ``esn0_db`` (``_demap.py:143-163``).
+
+    ``N0 = Es / 10^(EsN0/10)`` with ``Es`` the constellation's power; real
+    symbols carry the in-phase half, ``N0 / 2``.
+    """
+    from .._array import require_finite
+
+    if (noise_var is None) == (esn0_db is None):
+        raise ValueError(f"{function_name}: give one of noise_var and esn0_db.")
+    if noise_var is not None:
+        return require_finite(noise_var, f"{function_name}: noise_var", low=0.0)
+    snr = require_finite(esn0_db, f"{function_name}: esn0_db")
+    n0 = constellation.power() / 10 ** (snr / 10)
+    return n0 / 2 if real else n0
+
+
+def check_method(method: str) -> None:
+    if method not in ("maxlog", "exact"):
+        raise ValueError(f"Unknown method: {method}. Use 'maxlog' or 'exact'.")
+
+
+def log_prior(pmf: np.ndarray | None, order: int) -> np.ndarray:
+    """float32 log prior, ``log(clip(pmf, 1e-40))``; zeros for uniform (``_demap.py:220-224``)."""
+    if pmf is None:
+        return np.zeros(order, dtype=np.float32)
+    return np.log(np.clip(np.asarray(pmf, np.float64), 1e-40, None)).astype(np.float32)
+
+
+def opposite_levels(labels: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Per bit and level, the nearest level below / above with the other bit value.
+
+    ``labels`` is ``(L, k)`` over ascending levels; returns int32 ``lo``,
+    ``hi`` of shape ``(k, L)``, -1 where there is none.  These are the levels
+    just outside the run of equal bits holding each level
+    (``_demap.py:177-197``).
+    """
+    L, k = labels.shape
+    lo = np.full((k, L), -1, dtype=np.int32)
+    hi = np.full((k, L), -1, dtype=np.int32)
+    for b in range(k):
+        col = labels[:, b]
+        for m in range(L):
+            j = m - 1
+            while j >= 0 and col[j] == col[m]:
+                j -= 1
+            lo[b, m] = j
+            j = m + 1
+            while j < L and col[j] == col[m]:
+                j += 1
+            hi[b, m] = j if j < L else -1
+    return lo, hi
+
+
+class CkAxis(ctypes.Structure):
+    """``ck_axis`` of ``include/ck_tail.h``: one axis's device tables."""
+
+    _fields_ = [
+        ("levels", ctypes.c_void_p),
+        ("log_prior", ctypes.c_void_p),
+        ("labels", ctypes.c_void_p),
+        ("lo", ctypes.c_void_p),
+        ("hi", ctypes.c_void_p),
+        ("pos", ctypes.c_void_p),
+        ("num_levels", ctypes.c_int32),
+        ("num_bits", ctypes.c_int32),
+    ]
+
+
+def _axis_tables(a: Axis) -> tuple[np.ndarray, ...]:
+    lo, hi = opposite_levels(a.labels)
+    return (
+        a.levels.astype(np.float32),
+        log_prior(a.pmf, a.levels.size),
+        pack_bits(a.labels).astype(np.int32),
+        lo,
+        hi,
+        a.bits.astype(np.int32),
+    )
+
+
+def _ck_axis(t: tuple[Any, ...]) -> CkAxis:
+    lev, lp, lab, lo, hi, pos = t
+    return CkAxis(
+        lev.data_ptr(), lp.data_ptr(), lab.data_ptr(), lo.data_ptr(), hi.data_ptr(),
+        pos.data_ptr(), int(lev.numel()), int(pos.numel()),
+    )  # fmt: skip
+
+
+def axis_tables(c: "Constellation", device: Any) -> tuple[CkAxis, CkAxis, tuple]:
+    """The two ``ck_axis`` structs of a separable constellation on ``device``."""
+    re, im = c._axes  # type: ignore[misc]
+    t = device_tables(c, "axes", device, lambda: _axis_tables(re) + _axis_tables(im))
+    return _ck_axis(t[:6]), _ck_axis(t[6:]), t
+
+
+def closed_form(axes: tuple[Axis, Axis], method: str) -> bool:
+    """Max-log of a uniform separable constellation: the closed form."""
+    return method == "maxlog" and all(a.pmf is None and a.levels.size >= 2 for a in axes)
+
+
+def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
+    """LLRs of device symbols ``x`` (any shape ``(..., N)``), float32 ``(..., N*k)``."""
+    import torch
+
+    check_method(method)
+    k = c.bits_per_symbol
+    if k > LLR_MAX_K:
+        raise ValueError(
+            f"compute_llr(): GPU input supports at most {LLR_MAX_K} bits "
+            f"per symbol, the constellation has {k}"
+        )
+    real_input = not x.is_complex()
+    nv = 2.0 * noise_var if real_input else noise_var
+    inv_s2 = float(np.float32(1.0 / max(nv, 1e-20)))
+    shape = x.shape
+    flat = x.reshape(-1)
+    flat = flat.to(torch.complex64) if flat.dtype != torch.complex64 else flat.contiguous()
+    n = flat.numel()
+    out = torch.empty((n, k), dtype=torch.float32, device=x.device)
+    lib = _native.lib()
+    stream = _native.stream_of(x.device)
+    if n > 0:
+        with torch.cuda.device(x.device):
+            if c._axes is not None:
+                re_ax, im_ax, _keep = axis_tables(c, x.device)
+                if closed_form(c._axes, method):
+                    _native.check(lib.ck_llr_separable_maxlog(
+                        flat.data_ptr(), n, ctypes.byref(re_ax), ctypes.byref(im_ax),
+                        ctypes.c_float(inv_s2), out.data_ptr(), stream,
+                    ))  # fmt: skip
+                else:
+                    _native.check(lib.ck_llr_separable(
+                        flat.data_ptr(), n, ctypes.byref(re_ax), ctypes.byref(im_ax),
+                        ctypes.c_float(inv_s2), int(method == "exact"), out.data_ptr(),
+                        stream,
+                    ))  # fmt: skip
+            else:
+                pts, lp, lab = device_tables(
+                    c, "points", x.device,
+                    lambda: (
+                        c.points.astype(np.complex64),
+                        log_prior(c.pmf, c.order),
+                        pack_bits(c.bit_labels).astype(np.int32),
+                    ),
+                )  # fmt: skip
+                _native.check(lib.ck_llr_points(
+                    flat.data_ptr(), n, pts.data_ptr(), lp.data_ptr(), lab.data_ptr(),
+                    c.order, k, ctypes.c_float(inv_s2), int(method == "exact"),
+                    out.data_ptr(), stream,
+                ))  # fmt: skip
+    return out.reshape(*shape[:-1], shape[-1] * k) if len(shape) else out.reshape(-1)

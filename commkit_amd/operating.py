@@ -1,0 +1,100 @@
+"""Operating points of the seeded codes: the waterfall Es/N0.
+
+The workloads place frames relative to a code's waterfall ("0.3 dB above
+the waterfall", "uniform in [waterfall - 0.5, waterfall + 1] dB").  The
+reference has no such quantity in Es/N0 (its nearest tools are the EXIT
+threshold ``commkit/coding/exit.py:720-780``, the BICM GMI
+``commkit/metrics/awgn.py:477`` and Monte-Carlo error-rate curves
+``commkit/coding/performance.py:172-375``), so this module fixes one
+operational definition and measures it with the library's own decoder:
+
+    wf(code, constellation) = the lowest Es/N0 on a 0.05 dB grid at which
+    at most 1 % of 1024 seeded frames fail to decode (layered normalized
+    min-sum, scale 0.875, at most 50 iterations, early stop), i.e. the
+    frame-error-rate 1e-2 point of the Monte-Carlo curve.
+
+Frames are made by :func:`commkit_amd.workloads.make_frames` (data seed
+``2000 + grid index``) and demapped with max-log LLRs.  The measured
+values are kept in :data:`WATERFALL_DB`; :func:`calibrate` recomputes one
+on a GPU (``python -m commkit_amd.operating``).
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Any
+
+__all__ = ["WATERFALL_DB", "waterfall_db", "calibrate", "frame_error_rate"]
+
+#: Measured waterfalls (dB) by (code name, constellation order).
+WATERFALL_DB: dict[tuple[str, int], float] = {}
+
+_GRID = 0.05
+_FRAMES = 1024
+_TARGET_FER = 1e-2
+
+
+def waterfall_db(code_name: str, order: int) -> float:
+    try:
+        return WATERFALL_DB[(code_name, order)]
+    except KeyError:
+        raise KeyError(
+            f"no measured waterfall for {code_name} with {order}-QAM; run "
+            "`python -m commkit_amd.operating` on a GPU"
+        ) from None
+
+
+def frame_error_rate(code: Any, constellation: Any, esn0_db: float, frames: int, seed: int,
+                     method: Any = None) -> float:
+    """FER of ``frames`` seeded frames at ``esn0_db`` (GPU)."""
+    import torch
+
+    from .coding.ldpc import MinSum, _decode
+    from .mapping.demap import llr_any
+    from .workloads import make_frames
+
+    method = method or MinSum(max_iterations=50)
+    fr = make_frames(code, constellation, esn0_db, frames, seed=seed)
+    llr = llr_any(constellation, fr.rx, float(fr.noise_var[0]), "maxlog")
+    bits, _, _, _ = _decode(code, llr.reshape(frames, code.n), method)
+    wrong = (bits != fr.info).any(dim=1)
+    torch.cuda.synchronize()
+    return float(wrong.float().mean())
+
+
+def calibrate(code_name: str, order: int, *, start_db: float | None = None,
+              verbose: bool = True) -> float:
+    """Measure the waterfall of a seeded code with Gray ``order``-QAM."""
+    from .mapping.constellation import Constellation
+    from .workloads import seeded_code
+
+    code = seeded_code(code_name)
+    c = Constellation.qam(order)
+    # Start near the BICM-free AWGN capacity of the rate in bits per symbol.
+    bits = code.rate * math.log2(order)
+    es = 10 * math.log10(2**bits - 1) if start_db is None else start_db
+    es = round(es / _GRID) * _GRID
+
+    def fer(e: float) -> float:
+        idx = int(round(e / _GRID))
+        f = frame_error_rate(code, c, e, _FRAMES, seed=2000 + idx % 100000)
+        if verbose:
+            print(f"  {code_name} {order}-QAM Es/N0 {e:6.2f} dB: FER {f:.4f}", flush=True)
+        return f
+
+    # Coarse: rise in 0.25 dB steps until the target holds; then back off on the fine grid.
+    while fer(es) > _TARGET_FER:
+        es += 0.25
+    while es - _GRID > -50 and fer(es - _GRID) <= _TARGET_FER:
+        es -= _GRID
+    return round(es, 2)
+
+
+if __name__ == "__main__":
+    import sys
+
+    pairs = [("r4/5-z360", 64), ("r1/2-z27", 16)]
+    if len(sys.argv) > 1:
+        pairs = [(a.split(":")[0], int(a.split(":")[1])) for a in sys.argv[1:]]
+    for name, order in pairs:
+        print(f"waterfall {name} {order}-QAM: {calibrate(name, order)} dB", flush=True)

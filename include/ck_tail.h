@@ -1,0 +1,173 @@
+/*
+ * ck_tail.h - C ABI of libck_hip.so, the MI355X (gfx950) soft-decision
+ * receive tail: bit-LLR demapping and QC-LDPC decoding in hand-written HIP.
+ *
+ * Conventions (all entry points):
+ *   - plain C types only; every pointer named "device" is device memory of
+ *     the current HIP device (hipMalloc / torch tensor storage);
+ *   - `stream` is a hipStream_t (NULL = the null stream); work is enqueued,
+ *     nothing synchronises and nothing is copied device->host;
+ *   - the return value is 0 on success, else a ck_status; the message of the
+ *     last failure on the calling thread is ck_last_error();
+ *   - arguments are validated before any launch; a zero-size problem
+ *     returns 0 without launching (reference: commkit/_cuda/llr.py:24-25,
+ *     commkit/_cuda/coding.py:54-55).
+ *
+ * LLR sign convention: LLR = log P(b=0|y) / P(b=1|y); the k bits of a symbol
+ * are adjacent in the output, MSB first (commkit/mapping/llr.py:24-95).
+ */
+#ifndef CK_TAIL_H
+#define CK_TAIL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ck_status {
+    CK_OK = 0,
+    CK_EINVAL = 1,   /* bad argument (shape, size, null pointer, range) */
+    CK_EHIP = 2,     /* HIP runtime error (launch, attribute) */
+    CK_ENOSPACE = 3  /* problem exceeds a fixed limit of the library */
+};
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int32_t ck_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* ck_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Demapper
+ * ------------------------------------------------------------------------ */
+
+/* One axis of a separable square lattice (Gray QAM, shaped or not); all
+ * pointers are device memory.  Reference: the per-axis tables built by
+ * commkit/mapping/_demap.py:411-441 (levels, log prior, packed labels,
+ * opposite levels, bit positions). */
+typedef struct ck_axis {
+    const float* levels;     /* (L,) ascending, equally spaced for max-log */
+    const float* log_prior;  /* (L,) log marginal prior; zeros = uniform   */
+    const int32_t* labels;   /* (L,) label of each level over the axis's
+                                num_bits bits, MSB first                  */
+    const int32_t* lo;       /* (num_bits, L) nearest level below whose bit
+                                differs, -1 for none (closed form only)   */
+    const int32_t* hi;       /* (num_bits, L) nearest level above, -1 none */
+    const int32_t* pos;      /* (num_bits,) position of each bit in the
+                                symbol label                              */
+    int32_t num_levels;      /* L = 2^num_bits, 2..256                     */
+    int32_t num_bits;        /* 1..8                                       */
+} ck_axis;
+
+/* Closed-form max-log LLRs of a uniform separable lattice.
+ * x: (n,) complex64 device; out: (n, k) float32 device, k = re.num_bits +
+ * im.num_bits.  inv_s2 = 1 / sigma^2.
+ * Replaces the kernel `llr_separable_maxlog` (commkit/_cuda/src/llr.cu:
+ * 304-337) reached through _cuda.get_kernel("llr_separable_maxlog")
+ * (commkit/_cuda/llr.py:150; call site commkit/mapping/_demap.py:425). */
+int32_t ck_llr_separable_maxlog(const void* x, int64_t n, const ck_axis* re,
+                                const ck_axis* im, float inv_s2, float* out,
+                                void* stream);
+
+/* LLRs from the metrics of each axis's levels, with the log prior:
+ * exact = 0 max-log, 1 exact log-sum-exp.  Same layout as above.
+ * Replaces `llr_separable` (commkit/_cuda/src/llr.cu:225-259; wrapper
+ * commkit/_cuda/llr.py:87; call site commkit/mapping/_demap.py:444). */
+int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
+                         const ck_axis* im, float inv_s2, int32_t exact,
+                         float* out, void* stream);
+
+/* LLRs from the metrics of all M points (any constellation).
+ * points: (M,) complex64 device; log_prior: (M,) float32 device (zeros =
+ * uniform); labels: (M,) int32 packed MSB-first labels; 2 <= M <= 65536,
+ * k = log2(M) <= 16.  out: (n, k) float32.
+ * Replaces `llr` (commkit/_cuda/src/llr.cu:200-223; wrapper
+ * commkit/_cuda/llr.py; call site commkit/mapping/_demap.py:305). */
+int32_t ck_llr_points(const void* x, int64_t n, const void* points,
+                      const float* log_prior, const int32_t* labels, int32_t M,
+                      int32_t k, float inv_s2, int32_t exact, float* out,
+                      void* stream);
+
+/* ------------------------------------------------------------------------
+ * LDPC decoder
+ * ------------------------------------------------------------------------ */
+
+/* An LDPC code on the device.  The CSR view is always present; the
+ * quasi-cyclic view (lifting > 0) lets the decoder compute columns from
+ * circulant shifts instead of reading them.  Reference: LDPCCode._schedule
+ * (commkit/coding/ldpc.py:283-296) gives (row_ptr, cols, layer_ptr); the
+ * QC view restates LDPCCode.quasi_cyclic (commkit/coding/ldpc.py:183-214). */
+typedef struct ck_ldpc_code {
+    int32_t num_checks;        /* m                                          */
+    int32_t num_variables;     /* N (mother codeword)                        */
+    int32_t n;                 /* transmitted bits, N - punctured            */
+    int32_t k;                 /* information bits                           */
+    int32_t num_edges;         /* E                                          */
+    int32_t num_layers;        /* L                                          */
+    int32_t max_row_degree;    /* largest check degree                       */
+    const int32_t* row_ptr;    /* (m+1) CSR, rows in layer order             */
+    const int32_t* cols;       /* (E) ascending within a row                 */
+    const int32_t* layer_ptr;  /* (L+1) first row of each layer              */
+    const int32_t* var_ptr;    /* (N+1) edges of each variable ...           */
+    const int32_t* var_edges;  /* (E) ... as CSR edge ids, ascending check id
+                                  (flooding only; may be NULL otherwise)     */
+    const int32_t* info;       /* (k) codeword position of each info bit, or
+                                  NULL for positions 0..k-1                  */
+    int32_t lifting;           /* Z > 0 for a QC view, 0 for none            */
+    int32_t base_rows;         /* mb                                         */
+    int32_t base_cols;         /* nb                                         */
+    int32_t num_slots;         /* S = nonzero base entries                   */
+    const int32_t* qc_row_ptr; /* (mb+1) slots of each block row, block rows
+                                  in schedule order                          */
+    const int32_t* qc_slot_col;   /* (S) block column of each slot           */
+    const int32_t* qc_slot_shift; /* (S) circulant shift of each slot        */
+    const int32_t* qc_row_id;     /* (mb) natural index of each scheduled
+                                     block row                               */
+    const int32_t* qc_col_ptr;    /* (nb+1) slots of each block column ...   */
+    const int32_t* qc_col_slot;   /* (S) ... in ascending natural block row  */
+    const int32_t* qc_slot_pos;   /* (S) position of the slot in its row     */
+    const int32_t* qc_slot_row;   /* (S) scheduled block row of each slot    */
+} ck_ldpc_code;
+
+/* Decoding method.  rule/scale/offset/max_iterations/early_stop follow
+ * MinSum and SumProduct (commkit/coding/ldpc.py:317-371).  schedule 1
+ * (flooding) has no counterpart in the reference.  The tile fields are
+ * autotuner choices; 0 / -1 let the library choose. */
+typedef struct ck_ldpc_method {
+    int32_t rule;             /* 0 normalized/offset min-sum, 1 sum-product */
+    int32_t schedule;         /* 0 layered, 1 flooding                      */
+    float scale;              /* min-sum normalisation (1 for sum-product)  */
+    float offset;             /* min-sum offset (0 for sum-product)         */
+    int32_t max_iterations;   /* >= 1                                       */
+    int32_t early_stop;       /* stop a frame on a zero syndrome            */
+    int32_t kernel;           /* 0 auto, 1 generic CSR, 2 quasi-cyclic       */
+    int32_t frames_per_group; /* QC: frames per workgroup, 0 = auto         */
+    int32_t lanes_per_row;    /* QC min-sum: lanes per check row 1/2/4, 0 = auto */
+    int32_t state_in_lds;     /* QC min-sum: check state in LDS 1, HBM 0, -1 auto */
+} ck_ldpc_method;
+
+/* Device workspace (bytes) ck_ldpc_decode needs for R frames. */
+int64_t ck_ldpc_workspace_bytes(const ck_ldpc_code* code,
+                                const ck_ldpc_method* method, int64_t R);
+
+/* Decode R frames.  llrs (R, n) float32 channel LLRs in; app (R, n) float32
+ * a-posteriori LLRs out (may alias llrs); bits (R, k) int8 hard decisions of
+ * the info bits out (NULL to skip); iterations (R) int32 and valid (R)
+ * uint8 out; workspace: ck_ldpc_workspace_bytes() device bytes, 256-byte
+ * aligned (NULL when that is 0).  Punctured bits start at LLR 0.
+ * Replaces the kernel `ldpc_layered<SUM_PRODUCT>` (commkit/_cuda/src/
+ * ldpc.cu:107-159) reached through _cuda.get_kernel("ldpc_layered",
+ * sum_product=) (commkit/_cuda/coding.py:11-69; call site
+ * commkit/coding/ldpc.py:566-581), including the hard decisions of
+ * commkit/coding/ldpc.py:582-587. */
+int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method* method,
+                       const float* llrs, float* app, int8_t* bits,
+                       int32_t* iterations, uint8_t* valid, int64_t R,
+                       void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CK_TAIL_H */

@@ -1,0 +1,145 @@
+"""ctypes wrappers of ``libck_oracle.so`` -- TEST INFRASTRUCTURE ONLY.
+
+NumPy in, NumPy out.  Each wrapper names the C function (and through it the
+reference lines it restates, listed in ``oracle/ck_oracle.c``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libck_oracle.so")
+
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f32 = ctypes.c_float
+
+_SIGS = {
+    "oracle_ldpc_layered": [_p, _i64, _i32, _p, _p, _i32, _i32, _f32, _f32, _i32, _i32, _p, _p],
+    "oracle_ldpc_flooding": [_p, _i64, _i32, _p, _p, _i32, _p, _p, _i32, _f32, _f32, _i32, _i32,
+                             _p, _p],
+    "oracle_llr_points": [_p, _p, _i64, _p, _p, _p, _p, _i32, _i32, _f32, _i32, _p],
+    "oracle_llr_separable": [_p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _i32, _f32,
+                             _i32, _p],
+    "oracle_llr_separable_maxlog": [_p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32,
+                                    _i32, _f32, _p],
+}  # fmt: skip
+
+_LIB: ctypes.CDLL | None = None
+
+
+def build() -> str:
+    """Run oracle/Makefile (gcc); returns the library path."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = None
+            fn.argtypes = args
+        _LIB = h
+    return _LIB
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def _i32a(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f32a(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def ldpc_layered(app: np.ndarray, row_ptr, cols, *, sum_product: bool, scale: float = 1.0,
+                 offset: float = 0.0, max_iterations: int, early_stop: bool):
+    """Layered decoding of ``app`` (R, N) float32 in place; returns (iters, valid)."""
+    app = np.ascontiguousarray(app, dtype=np.float32)
+    row_ptr, cols = _i32a(row_ptr), _i32a(cols)
+    R, N = app.shape
+    iters = np.empty(R, np.int32)
+    valid = np.empty(R, np.uint8)
+    lib().oracle_ldpc_layered(
+        _ptr(app), R, N, _ptr(row_ptr), _ptr(cols), row_ptr.size - 1, int(sum_product),
+        float(np.float32(scale)), float(np.float32(offset)), max_iterations, int(early_stop),
+        _ptr(iters), _ptr(valid),
+    )  # fmt: skip
+    return app, iters, valid.astype(bool)
+
+
+def ldpc_flooding(app: np.ndarray, row_ptr, cols, var_ptr, var_edges, *, sum_product: bool,
+                  scale: float = 1.0, offset: float = 0.0, max_iterations: int,
+                  early_stop: bool):
+    app = np.ascontiguousarray(app, dtype=np.float32)
+    row_ptr, cols, var_ptr, var_edges = map(_i32a, (row_ptr, cols, var_ptr, var_edges))
+    R, N = app.shape
+    iters = np.empty(R, np.int32)
+    valid = np.empty(R, np.uint8)
+    lib().oracle_ldpc_flooding(
+        _ptr(app), R, N, _ptr(row_ptr), _ptr(cols), row_ptr.size - 1, _ptr(var_ptr),
+        _ptr(var_edges), int(sum_product), float(np.float32(scale)), float(np.float32(offset)),
+        max_iterations, int(early_stop), _ptr(iters), _ptr(valid),
+    )  # fmt: skip
+    return app, iters, valid.astype(bool)
+
+
+def half_indices(bit_labels: np.ndarray) -> np.ndarray:
+    """(2, k, M/2) int32: the points whose bit b is 0 and 1."""
+    k = bit_labels.shape[1]
+    return np.stack(
+        [np.stack([np.flatnonzero(bit_labels[:, b] == v) for b in range(k)]) for v in (0, 1)]
+    ).astype(np.int32)
+
+
+def llr_points(x: np.ndarray, points: np.ndarray, bit_labels: np.ndarray, log_pmf: np.ndarray,
+               inv_s2: float, exact: bool) -> np.ndarray:
+    xr, xi = _f32a(np.real(x).ravel()), _f32a(np.imag(x).ravel())
+    M, k = bit_labels.shape
+    out = np.empty((xr.size, k), np.float32)
+    cr, ci, lp = _f32a(np.real(points)), _f32a(np.imag(points)), _f32a(log_pmf)
+    idx = half_indices(bit_labels)
+    lib().oracle_llr_points(_ptr(xr), _ptr(xi), xr.size, _ptr(cr), _ptr(ci), _ptr(lp), _ptr(idx),
+                            M, k, float(np.float32(inv_s2)), int(exact), _ptr(out))  # fmt: skip
+    return out.reshape(-1)
+
+
+def llr_separable(x: np.ndarray, axes, inv_s2: float, exact: bool) -> np.ndarray:
+    """``axes``: two (levels, labels, log_prior, bits) tuples (host tables)."""
+    xr, xi = _f32a(np.real(x).ravel()), _f32a(np.imag(x).ravel())
+    (lre, labre, lpre, bre), (lim, labim, lpim, bim) = axes
+    L, kb = labre.shape
+    out = np.empty((xr.size, 2 * kb), np.float32)
+    t = [_f32a(lre), _f32a(lim), _f32a(lpre), _f32a(lpim), half_indices(labre),
+         half_indices(labim), _i32a(bre), _i32a(bim)]  # fmt: skip
+    lib().oracle_llr_separable(_ptr(xr), _ptr(xi), xr.size, *[_ptr(a) for a in t], L, kb,
+                               float(np.float32(inv_s2)), int(exact), _ptr(out))  # fmt: skip
+    return out.reshape(-1)
+
+
+def llr_separable_maxlog(x: np.ndarray, axes, inv_s2: float) -> np.ndarray:
+    """``axes``: two (levels, packed labels, lo, hi, pos) tuples."""
+    xr, xi = _f32a(np.real(x).ravel()), _f32a(np.imag(x).ravel())
+    (lre, labre, lore, hire, pre), (lim, labim, loim, hiim, pim) = axes
+    L, kb = len(lre), len(pre)
+    out = np.empty((xr.size, 2 * kb), np.float32)
+    t = [_f32a(lre), _i32a(labre), _i32a(lore), _i32a(hire), _i32a(pre), _f32a(lim),
+         _i32a(labim), _i32a(loim), _i32a(hiim), _i32a(pim)]  # fmt: skip
+    lib().oracle_llr_separable_maxlog(_ptr(xr), _ptr(xi), xr.size, *[_ptr(a) for a in t], L, kb,
+                                      float(np.float32(inv_s2)), _ptr(out))  # fmt: skip
+    return out.reshape(-1)

@@ -1,0 +1,137 @@
+"""HIP demapper against the FP64 definition and the float32 CPU oracle (GPU).
+
+Tolerance (stated here, as the LLR contract): an LLR L32 from the GPU meets
+``|L32 - L64| <= 1e-5 * max(|L64|, floor)`` with L64 the float64 LLR of the
+definition, and ``floor = 2^10 * s`` where ``s`` bounds what float32 input
+rounding alone does to a metric difference (``oracle.llr64.error_scale``).
+So LLRs are within relative 1e-5 except where they are themselves within a
+few thousand roundings of zero; each case prints how many fell under the
+floor.  Where |L64| exceeds the floor, the sign (the hard decision) must
+agree.
+"""
+
+import numpy as np
+import pytest
+
+from commkit_amd import Constellation, Signal, compute_llr
+from commkit_amd.mapping.demap import log_prior, opposite_levels
+from commkit_amd.mapping.constellation import pack_bits
+from oracle import cpu
+from oracle.llr64 import error_scale, llr_fp64, llr_tolerance_ok
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "qpsk": (lambda: Constellation.qam(4), 0.3),
+    "16qam": (lambda: Constellation.qam(16), 10 ** (-0.8)),
+    "64qam": (lambda: Constellation.qam(64), 10 ** (-1.4)),
+    "256qam": (lambda: Constellation.qam(256), 10 ** (-2.0)),
+    "64qam-shaped": (lambda: Constellation.qam(64).shaped(nu=0.05), 10 ** (-1.3)),
+    "256qam-shaped": (lambda: Constellation.qam(256).shaped(nu=0.05), 10 ** (-1.8)),
+    "8psk": (lambda: Constellation.psk(8), 0.05),
+    "8qam": (lambda: Constellation.qam(8), 0.05),
+}
+
+
+def _noisy(c, n, nv, seed):
+    r = np.random.default_rng(seed)
+    if c.pmf is not None:
+        idx = r.choice(c.order, n, p=c.pmf)
+    else:
+        idx = r.integers(0, c.order, n)
+    noise = np.sqrt(nv / 2) * (r.standard_normal(n) + 1j * r.standard_normal(n))
+    return (c.points[idx] + noise).astype(np.complex64)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("method", ["maxlog", "exact"])
+def test_llr_against_fp64(gpu, name, method):
+    import torch
+
+    make, nv = CASES[name]
+    c = make()
+    x = _noisy(c, 20000, nv, seed=11)
+    got = compute_llr(torch.from_numpy(x).cuda(), noise_var=nv, constellation=c, method=method)
+    assert got.dtype == torch.float32 and got.shape == (x.size * c.bits_per_symbol,)
+    got = got.cpu().numpy()
+    ref = llr_fp64(x, c.points, c.bit_labels, c.pmf, nv, method)
+    scale = error_scale(x, c.points, nv)
+    ok, under = llr_tolerance_ok(got, ref, scale, c.bits_per_symbol)
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
+    print(f"{name} {method}: max rel err above floor {rel[~under].max():.2e}, "
+          f"{int(under.sum())} of {ref.size} under the floor")  # fmt: skip
+    assert ok, f"max rel err {rel[~under].max():.3e}"
+    assert np.array_equal(np.sign(got[~under]), np.sign(ref[~under]))
+
+
+@pytest.mark.parametrize("order", [16, 64, 256])
+def test_llr_against_c_oracle(gpu, order):
+    import torch
+
+    c = Constellation.qam(order)
+    nv = 0.02
+    x = _noisy(c, 5000, nv, seed=12)
+    re, im = c._axes
+    inv = float(np.float32(1 / nv))
+    closed = []
+    for a in (re, im):
+        lo, hi = opposite_levels(a.labels)
+        closed.append((a.levels, pack_bits(a.labels), lo, hi, a.bits))
+    want = cpu.llr_separable_maxlog(x, closed, inv)
+    got = compute_llr(torch.from_numpy(x).cuda(), noise_var=nv, constellation=c).cpu().numpy()
+    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-3)
+    sep = [(a.levels, a.labels, log_prior(a.pmf, a.levels.size), a.bits) for a in (re, im)]
+    want = cpu.llr_separable(x, sep, inv, True)
+    got = compute_llr(torch.from_numpy(x).cuda(), noise_var=nv, constellation=c,
+                      method="exact").cpu().numpy()  # fmt: skip
+    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-3)
+
+
+def test_noiseless_signs_at_scale(gpu):
+    import torch
+
+    c = Constellation.qam(64)
+    r = np.random.default_rng(13)
+    n = 1 << 22
+    idx = torch.from_numpy(r.integers(0, 64, n)).cuda()
+    pts = torch.from_numpy(c.points.astype(np.complex64)).cuda()
+    x = pts[idx]
+    for method in ("maxlog", "exact"):
+        llr = compute_llr(x, noise_var=0.05, constellation=c, method=method)
+        labels = torch.from_numpy(c.bit_labels).cuda()[idx].reshape(-1)
+        assert torch.equal((llr < 0).to(torch.int8), labels)
+
+
+def test_contract(gpu):
+    import torch
+
+    c = Constellation.qam(16)
+    x = _noisy(c, 15, 0.1, seed=14).reshape(3, 5)
+    host = compute_llr(x, noise_var=0.1, constellation=c)
+    assert isinstance(host, np.ndarray) and host.shape == (3, 20) and host.dtype == np.float32
+    dev = compute_llr(torch.from_numpy(x).cuda(), noise_var=0.1, constellation=c)
+    assert dev.device.type == "cuda" and dev.shape == (3, 20)
+    np.testing.assert_array_equal(dev.cpu().numpy(), host)
+    sig = compute_llr(Signal(x, constellation=c), esn0_db=10.0)
+    np.testing.assert_array_equal(sig, host)
+    via = c.llr(x, noise_var=0.1)
+    np.testing.assert_array_equal(via, host)
+    empty = compute_llr(np.zeros(0, np.complex64), noise_var=0.1, constellation=c)
+    assert empty.shape == (0,)
+    with pytest.raises(ValueError, match="sps"):
+        compute_llr(Signal(x, constellation=c, sps=2), noise_var=0.1)
+    from commkit_amd import PrecisionWarning
+
+    with pytest.warns(PrecisionWarning):
+        compute_llr(x.astype(np.complex128), noise_var=0.1, constellation=c)
+
+
+def test_real_pam(gpu):
+    c = Constellation.pam(4)
+    r = np.random.default_rng(15)
+    x = (c.points[r.integers(0, 4, 3000)] + 0.2 * r.standard_normal(3000)).astype(np.float32)
+    nv = 0.04  # real noise variance
+    got = compute_llr(x, noise_var=nv, constellation=c)
+    ref = llr_fp64(x, c.points, c.bit_labels, None, nv, "maxlog")
+    ok, under = llr_tolerance_ok(got, ref, error_scale(x, c.points, 2 * nv), 2)
+    assert ok

@@ -1,0 +1,219 @@
+"""The CPU oracle pinned against mathematics and the reference's known answers.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign, a swapped operand, a wrong tie rule) fails it:
+
+- C float32 LLRs against the FP64 definition (brute force over all points);
+- the Gray QPSK closed form |LLR| = 2 sqrt(2) |y| / sigma^2
+  (``tests/mapping/test_llr.py:121-148`` of the reference);
+- noiseless symbols give the sign of the transmitted bit;
+- the closed-form max-log equals the level metrics bit for bit
+  (``tests/mapping/test_llr.py:467-484``);
+- layered min-sum / sum-product against the FP64 textbook rule
+  (``tests/coding/test_ldpc.py:186-217``: atol 1e-4 / 5e-3);
+- on a cycle-free code, sum-product = bitwise MAP and min-sum (scale 1) =
+  max-log MAP by enumeration (``tests/coding/test_ldpc.py:219-252``);
+- noiseless codewords stop at iteration 0, random LLRs fail with
+  iterations = max (``tests/coding/test_ldpc.py:157-165,254-261``);
+- flooding against its FP64 textbook form, and equal to layered on a code
+  whose checks share no bit.
+"""
+
+import numpy as np
+import pytest
+
+from commkit_amd.mapping import Constellation
+from commkit_amd.mapping.demap import log_prior, opposite_levels
+from commkit_amd.mapping.constellation import pack_bits
+from oracle import cpu
+from oracle.ldpc64 import flooding_fp64, layered_fp64, tree_map
+from oracle.llr64 import llr_fp64
+
+from .codes import encode_host, rows_in_schedule, small_qc, tree_code
+
+
+def _noisy(c: Constellation, n: int, nv: float, rng) -> np.ndarray:
+    idx = rng.integers(0, c.order, n)
+    pts = c.points[idx]
+    noise = np.sqrt(nv / 2) * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    return (pts + noise).astype(np.complex64)
+
+
+def _axes_tables(c: Constellation):
+    re, im = c._axes
+    sep = [(a.levels, a.labels, log_prior(a.pmf, a.levels.size), a.bits) for a in (re, im)]
+    closed = []
+    for a in (re, im):
+        lo, hi = opposite_levels(a.labels)
+        closed.append((a.levels, pack_bits(a.labels), lo, hi, a.bits))
+    return sep, closed
+
+
+@pytest.mark.parametrize("order", [4, 16, 64, 256])
+@pytest.mark.parametrize("method", ["maxlog", "exact"])
+def test_c_llr_matches_fp64_definition(order, method, rng):
+    c = Constellation.qam(order)
+    nv = 0.05 if order <= 16 else 0.01
+    x = _noisy(c, 3000, nv, rng)
+    ref = llr_fp64(x, c.points, c.bit_labels, c.pmf, nv, method)
+    inv = float(np.float32(1.0 / nv))
+    got_pts = cpu.llr_points(x, c.points, c.bit_labels, log_prior(None, c.order), inv,
+                             method == "exact")
+    sep, closed = _axes_tables(c)
+    got_sep = cpu.llr_separable(x, sep, inv, method == "exact")
+    scale = np.maximum(np.abs(ref), 1.0)
+    for got in (got_pts, got_sep):
+        assert np.all(np.abs(got - ref) <= 2e-4 * scale)
+        assert np.array_equal(np.sign(got[np.abs(ref) > 1e-2]), np.sign(ref[np.abs(ref) > 1e-2]))
+    if method == "maxlog":
+        got_closed = cpu.llr_separable_maxlog(x, closed, inv)
+        # Closed form = level metrics, bit for bit (both are the float32 metrics
+        # of the same two levels).
+        np.testing.assert_array_equal(got_closed, got_sep)
+
+
+def test_c_llr_shaped_prior_matches_fp64(rng):
+    c = Constellation.qam(64).shaped(nu=0.05)
+    nv = 0.02
+    x = _noisy(c, 2000, nv, rng)
+    for method in ("maxlog", "exact"):
+        ref = llr_fp64(x, c.points, c.bit_labels, c.pmf, nv, method)
+        sep, _ = _axes_tables(c)
+        got = cpu.llr_separable(x, sep, float(np.float32(1 / nv)), method == "exact")
+        assert np.all(np.abs(got - ref) <= 2e-4 * np.maximum(np.abs(ref), 1.0))
+
+
+def test_qpsk_closed_form(rng):
+    """Gray QPSK: |LLR| = 2 sqrt(2) |y| / sigma^2 per axis, exact = max-log."""
+    c = Constellation.qam(4)
+    nv = 0.3
+    x = _noisy(c, 500, nv, rng)
+    for method in ("maxlog", "exact"):
+        ref = llr_fp64(x, c.points, c.bit_labels, None, nv, method).reshape(-1, 2)
+        y = np.stack([x.real, x.imag], axis=1).astype(np.float64)
+        np.testing.assert_allclose(np.abs(ref), 2 * np.sqrt(2) * np.abs(y) / nv, rtol=1e-9)
+        # Gray QPSK puts label 0 at the negative level of each axis.
+        assert c.points[0].real < 0 and c.points[0].imag < 0
+        assert np.all((ref < 0) == (y > 0))
+
+
+def test_noiseless_signs_are_transmitted_bits(rng):
+    for c in (Constellation.qam(16), Constellation.psk(8), Constellation.qam(256)):
+        idx = rng.integers(0, c.order, 400)
+        x = c.points[idx].astype(np.complex64)
+        bits = c.bit_labels[idx].reshape(-1)
+        ref = llr_fp64(x, c.points, c.bit_labels, None, 0.1, "maxlog")
+        np.testing.assert_array_equal((ref < 0).astype(np.int8), bits)
+        got = cpu.llr_points(x, c.points, c.bit_labels, log_prior(None, c.order), 10.0, False)
+        np.testing.assert_array_equal((got < 0).astype(np.int8), bits)
+
+
+def test_opposite_levels_definition():
+    """lo/hi: nearest level below/above whose bit differs (brute force)."""
+    for order in (4, 16, 64, 256):
+        labels = Constellation.qam(order)._axes[0].labels
+        lo, hi = opposite_levels(labels)
+        L, k = labels.shape
+        for b in range(k):
+            for m in range(L):
+                below = [j for j in range(m) if labels[j, b] != labels[m, b]]
+                above = [j for j in range(m + 1, L) if labels[j, b] != labels[m, b]]
+                assert lo[b, m] == (max(below) if below else -1)
+                assert hi[b, m] == (min(above) if above else -1)
+
+
+# -- decoder ---------------------------------------------------------------
+
+
+def _llrs(n: int, rows: int, mean: float, seed: int) -> np.ndarray:
+    r = np.random.default_rng(seed)
+    return (mean + 1.5 * r.standard_normal((rows, n))).astype(np.float32)
+
+
+@pytest.mark.parametrize(
+    "sp,scale,offset", [(False, 0.75, 0.1), (False, 1.0, 0.0), (True, 1.0, 0.0)],
+    ids=["offset", "plain", "sum-product"],
+)  # fmt: skip
+def test_layered_matches_textbook(sp, scale, offset):
+    code = small_qc()
+    row_ptr, cols, _ = code._schedule
+    llrs = _llrs(code.n, 3, 1.0, 3)
+    got, _, _ = cpu.ldpc_layered(llrs.copy(), row_ptr, cols, sum_product=sp, scale=scale,
+                                 offset=offset, max_iterations=3, early_stop=False)  # fmt: skip
+    for b in range(3):
+        ref = layered_fp64(llrs[b], rows_in_schedule(code), sum_product=sp, scale=scale,
+                           offset=offset, iterations=3)  # fmt: skip
+        np.testing.assert_allclose(got[b], ref, atol=5e-3 if sp else 1e-4, rtol=1e-4)
+
+
+@pytest.mark.parametrize(
+    "sp,scale,offset", [(False, 0.75, 0.1), (True, 1.0, 0.0)], ids=["min-sum", "sum-product"]
+)
+def test_flooding_matches_textbook(sp, scale, offset):
+    code = small_qc()
+    row_ptr, cols, _ = code._schedule
+    var_ptr, var_edges = code._var_view
+    llrs = _llrs(code.n, 3, 1.0, 4)
+    got, _, _ = cpu.ldpc_flooding(llrs.copy(), row_ptr, cols, var_ptr, var_edges, sum_product=sp,
+                                  scale=scale, offset=offset, max_iterations=4,
+                                  early_stop=False)  # fmt: skip
+    for b in range(3):
+        ref = flooding_fp64(llrs[b], rows_in_schedule(code), sum_product=sp, scale=scale,
+                            offset=offset, iterations=4)  # fmt: skip
+        np.testing.assert_allclose(got[b], ref, atol=5e-3 if sp else 1e-4, rtol=1e-4)
+
+
+def test_tree_code_is_exact_both_schedules():
+    code = tree_code()
+    h = code.parity_check().toarray()
+    llrs = np.random.default_rng(4).standard_normal((5, 8)).astype(np.float32)
+    exact, maxlog = tree_map(llrs, h)
+    row_ptr, cols, _ = code._schedule
+    var_ptr, var_edges = code._var_view
+    for sp, ref in ((True, exact), (False, maxlog)):
+        lay, _, _ = cpu.ldpc_layered(llrs.copy(), row_ptr, cols, sum_product=sp,
+                                     max_iterations=10, early_stop=False)  # fmt: skip
+        fl, _, _ = cpu.ldpc_flooding(llrs.copy(), row_ptr, cols, var_ptr, var_edges,
+                                     sum_product=sp, max_iterations=10, early_stop=False)
+        np.testing.assert_allclose(lay, ref, atol=2e-5)
+        np.testing.assert_allclose(fl, ref, atol=2e-5)
+
+
+def test_flooding_equals_layered_when_checks_are_disjoint():
+    # Two checks on disjoint bits: one layer, flooding and layered agree.
+    from commkit_amd.coding.ldpc import LDPCCode
+
+    code = LDPCCode([0, 0, 0, 1, 1, 1], [0, 1, 2, 3, 4, 5], 2, 7)
+    row_ptr, cols, layer_ptr = code._schedule
+    assert layer_ptr.size == 2
+    var_ptr, var_edges = code._var_view
+    llrs = _llrs(7, 4, 0.5, 5)
+    lay, it1, _ = cpu.ldpc_layered(llrs.copy(), row_ptr, cols, sum_product=False, scale=0.8,
+                                   max_iterations=5, early_stop=False)  # fmt: skip
+    fl, it2, _ = cpu.ldpc_flooding(llrs.copy(), row_ptr, cols, var_ptr, var_edges,
+                                   sum_product=False, scale=0.8, max_iterations=5,
+                                   early_stop=False)  # fmt: skip
+    np.testing.assert_allclose(fl, lay, atol=1e-5)
+
+
+@pytest.mark.parametrize("flood", [False, True])
+def test_early_stop_semantics(flood):
+    code = small_qc()
+    row_ptr, cols, _ = code._schedule
+    var_ptr, var_edges = code._var_view
+    u = np.random.default_rng(7).integers(0, 2, (4, code.k))
+    words = encode_host(code, u)
+    clean = (1.0 - 2.0 * words).astype(np.float32) * 3.0
+    kw = dict(sum_product=False, scale=0.75, max_iterations=6, early_stop=True)
+
+    def run(x):
+        if flood:
+            return cpu.ldpc_flooding(x.copy(), row_ptr, cols, var_ptr, var_edges, **kw)
+        return cpu.ldpc_layered(x.copy(), row_ptr, cols, **kw)
+
+    app, iters, valid = run(clean)
+    assert valid.all() and (iters == 0).all()
+    np.testing.assert_array_equal(app, clean)
+    noise = np.random.default_rng(8).standard_normal((4, code.n)).astype(np.float32)
+    _, iters, valid = run(noise)
+    assert not valid.any() and (iters == 6).all()
