@@ -41,6 +41,8 @@ def _parse() -> argparse.Namespace:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--frames", type=int, default=FRAMES)
+    ap.add_argument("--esn0", type=float, default=None,
+                    help="override Es/N0 (dB); default: measured waterfall + 0.3 dB")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the baseline sample")
@@ -118,7 +120,7 @@ def main() -> int:
     torch.cuda.set_device(dev)
     code = seeded_code(CODE)
     c = Constellation.qam(ORDER)
-    esn0 = round(waterfall_db(CODE, ORDER) + MARGIN_DB, 3)
+    esn0 = args.esn0 if args.esn0 is not None else round(waterfall_db(CODE, ORDER) + MARGIN_DB, 3)
     F = args.frames
     fr = make_frames(code, c, esn0, F, seed=0, device=dev)
     nv = float(fr.noise_var[0])

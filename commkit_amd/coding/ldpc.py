@@ -267,6 +267,9 @@ class LDPCCode:
         col_slot = np.lexsort((nat, slot_col_a)).astype(np.int32)
         col_ptr = np.concatenate([[0], np.cumsum(np.bincount(slot_col_a, minlength=nb))])
         degree = np.diff(np.asarray(slot_ptr))
+        packed = np.zeros((len(slot_col) + 32, 2), dtype=np.int32)
+        packed[: len(slot_col), 0] = np.asarray(slot_col) * z
+        packed[: len(slot_col), 1] = slot_shift
         return {
             "lifting": np.int32(z),
             "base_rows": np.int32(mb),
@@ -280,6 +283,7 @@ class LDPCCode:
             "slot_pos": np.asarray(slot_pos, np.int32),
             "slot_row": np.asarray(slot_row, np.int32),
             "max_degree": np.int32(degree.max()),
+            "slot_packed": packed,
         }
 
     @cached_property
@@ -496,6 +500,7 @@ class CkLdpcCode(ctypes.Structure):
         ("qc_col_slot", ctypes.c_void_p),
         ("qc_slot_pos", ctypes.c_void_p),
         ("qc_slot_row", ctypes.c_void_p),
+        ("qc_slot_packed", ctypes.c_void_p),
     ]
 
 
@@ -535,7 +540,8 @@ def device_code(code: LDPCCode, device: Any) -> DeviceCode:
         base = [row_ptr, cols, layer_ptr, var_ptr, var_edges, info.astype(np.int32)]
         if qc is not None:
             base += [qc[key] for key in ("row_ptr", "slot_col", "slot_shift", "row_id",
-                                         "col_ptr", "col_slot", "slot_pos", "slot_row")]  # fmt: skip
+                                         "col_ptr", "col_slot", "slot_pos", "slot_row",
+                                         "slot_packed")]  # fmt: skip
         return tuple(np.ascontiguousarray(a, dtype=np.int32) for a in base)
 
     t = device_tables(code, "decoder", device, build)
@@ -557,7 +563,8 @@ def device_code(code: LDPCCode, device: Any) -> DeviceCode:
         s.base_cols = int(qc["base_cols"])
         s.num_slots = int(qc["slot_col"].size)
         (s.qc_row_ptr, s.qc_slot_col, s.qc_slot_shift, s.qc_row_id, s.qc_col_ptr,
-         s.qc_col_slot, s.qc_slot_pos, s.qc_slot_row) = (x.data_ptr() for x in t[6:14])  # fmt: skip
+         s.qc_col_slot, s.qc_slot_pos, s.qc_slot_row,
+         s.qc_slot_packed) = (x.data_ptr() for x in t[6:15])  # fmt: skip
     return DeviceCode(s, t)
 
 

@@ -158,10 +158,46 @@ __device__ __forceinline__ QcLane qc_lane(int Z, int F, long long R) {
     return l;
 }
 
-__device__ __forceinline__ int qc_col(int j, int r, int sh, int Z) {
+// Uniform table loads: through the constant address space, so a
+// wave-uniform index becomes a scalar (SMEM) load instead of a vector load
+// whose latency every edge of the row would wait on.
+#define CK_CONSTANT __attribute__((address_space(4)))
+
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p, int i) {
+    return ((const CK_CONSTANT T*)p)[i];
+}
+
+// {block column * Z, shift} of slots base + t*P + p, t = 0..G-1, for the lane
+// with row-lane index p.  The G*P entries are consecutive in the packed,
+// zero-padded slot table and read with uniform addresses (one wide scalar
+// load); each lane keeps its own.  Entries past a row's end are real table
+// entries or padding, so their addresses stay inside the frame's APP.
+template <int P, int G>
+__device__ __forceinline__ void slot_group(const ck_ldpc_code& c, int base, int p, int* jz,
+                                           int* sh) {
+    const CK_CONSTANT int* tab = (const CK_CONSTANT int*)c.qc_slot_packed;
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+        const int i0 = 2 * (base + t * P);
+        int vx = tab[i0];
+        int vy = tab[i0 + 1];
+#pragma unroll
+        for (int u = 1; u < P; ++u) {
+            const int wx = tab[i0 + 2 * u];
+            const int wy = tab[i0 + 2 * u + 1];
+            vx = (p == u) ? wx : vx;
+            vy = (p == u) ? wy : vy;
+        }
+        jz[t] = vx;
+        sh[t] = vy;
+    }
+}
+
+__device__ __forceinline__ int qc_addr(int jz, int r, int sh, int Z) {
     int col = r + sh;
     col -= (col >= Z) ? Z : 0;
-    return j * Z + col;
+    return jz + col;
 }
 
 // Whether every check of the lane's frame is satisfied; `test` says whether
@@ -172,13 +208,22 @@ __device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const float* a, const QcLa
     const int Z = c.lifting;
     int bad = 0;
     if (test) {
-        for (int br = l.p; br < c.base_rows; br += P) {
-            const int s0 = c.qc_row_ptr[br];
-            const int s1 = c.qc_row_ptr[br + 1];
-            int parity = 0;
-            for (int s = s0; s < s1; ++s)
-                parity ^= a[qc_col(c.qc_slot_col[s], l.r, c.qc_slot_shift[s], Z)] < 0.0f;
-            bad |= parity;
+        // Lane p of a row checks the rows of block rows p, p+P, ...
+        for (int br0 = 0; br0 < c.base_rows; br0 += P) {
+            const int br = min(br0 + l.p, c.base_rows - 1);
+            const int s0 = ld_uniform(c.qc_row_ptr, br);
+            const int s1 = ld_uniform(c.qc_row_ptr, br + 1);
+            unsigned parity = 0u;
+            for (int s = s0; s < s1; s += 4) {
+                float v[4];
+                int jz[4], sh[4];
+                slot_group<1, 4>(c, s, 0, jz, sh);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = a[qc_addr(jz[u], l.r, sh[u], Z)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) parity ^= (s + u < s1 && v[u] < 0.0f) ? 1u : 0u;
+            }
+            bad |= (br0 + l.p < c.base_rows) ? static_cast<int>(parity) : 0;
         }
     }
     if (F == 1) return !__syncthreads_or(bad);
@@ -191,38 +236,61 @@ __device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const float* a, const QcLa
     return ok;
 }
 
-// One min-sum check row of a block row (layered when UPDATE_APP, else the
-// flooding check phase, which leaves the APP alone).
+__device__ __forceinline__ CheckState zero_state() {
+    CheckState z;
+    z.m1 = 0.0f;
+    z.m2 = 0.0f;
+    z.signs = 0u;
+    z.idx = 0;
+    return z;
+}
+
+// One min-sum check row of a block row, given its old state; returns the new
+// state.  UPDATE_APP: layered (APP = Q + R_new); else the flooding check
+// phase, which leaves the APP alone.  Q and the LDS address of every edge of
+// the lane stay in registers between the two passes (MAXT = 32 / P slots).
 template <int P, bool UPDATE_APP>
-__device__ __forceinline__ void qc_ms_row(const ck_ldpc_code& c, float* a, CheckState* st_ptr,
-                                          int br, int r, int p, bool first, const Params& prm) {
+__device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, float* a,
+                                                const CheckState st, int br, int r, int p,
+                                                const Params& prm) {
+    constexpr int MAXT = kMaxDegree / P;
+    constexpr int G = 4;  // edges per group: a group's loads are issued together
     const int Z = c.lifting;
-    const int s0 = c.qc_row_ptr[br];
-    const int dc = c.qc_row_ptr[br + 1] - s0;
-    CheckState st;
-    if (first) {
-        st.m1 = 0.0f;
-        st.m2 = 0.0f;
-        st.signs = 0u;
-        st.idx = 0;
-    } else {
-        st = *st_ptr;
-    }
+    const int s0 = ld_uniform(c.qc_row_ptr, br);
+    const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
+    const int tmax = (dc + P - 1) / P;
+    float q[MAXT];
+    int addr[MAXT];
     float min1 = ck::kInf;
     float min2 = ck::kInf;
     int idx = 0x7fffffff;
     unsigned neg = 0u;
-    for (int e = p; e < dc; e += P) {
-        const int s = s0 + e;
-        const float q = a[qc_col(c.qc_slot_col[s], r, c.qc_slot_shift[s], Z)] - state_msg(st, e);
-        neg ^= (q < 0.0f) ? 1u : 0u;
-        const float mag = fabsf(q);
-        if (mag < min1) {
-            min2 = min1;
-            min1 = mag;
-            idx = e;
-        } else if (mag < min2) {
-            min2 = mag;
+#pragma unroll
+    for (int g = 0; g < MAXT; g += G) {
+        if (g < tmax) {
+            // Slots past the row's end are clamped to real table entries, so
+            // their (unused) addresses stay inside the frame's APP.
+            int jz[G], sh[G];
+            slot_group<P, G>(c, s0 + g * P, p, jz, sh);
+#pragma unroll
+            for (int t = g; t < g + G; ++t) {
+                addr[t] = qc_addr(jz[t - g], r, sh[t - g], Z);
+                q[t] = a[addr[t]];
+            }
+#pragma unroll
+            for (int t = g; t < g + G; ++t) {
+                const int e = t * P + p;
+                const bool valid = e < dc;
+                const float qv = q[t] - state_msg(st, e);
+                q[t] = qv;
+                const float mag = valid ? fabsf(qv) : ck::kInf;
+                const bool lt1 = mag < min1;
+                const bool lt2 = mag < min2;
+                min2 = lt1 ? min1 : (lt2 ? mag : min2);
+                idx = lt1 ? e : idx;
+                min1 = lt1 ? mag : min1;
+                neg ^= (valid && qv < 0.0f) ? 1u : 0u;
+            }
         }
     }
 #pragma unroll
@@ -242,54 +310,73 @@ __device__ __forceinline__ void qc_ms_row(const ck_ldpc_code& c, float* a, Check
     const float m1 = fmaxf(prm.scale * min1 - prm.offset, 0.0f);
     const float m2 = fmaxf(prm.scale * min2 - prm.offset, 0.0f);
     unsigned signs = 0u;
-    for (int e = p; e < dc; e += P) {
-        const int s = s0 + e;
-        const int addr = qc_col(c.qc_slot_col[s], r, c.qc_slot_shift[s], Z);
-        const float q = a[addr] - state_msg(st, e);
-        const unsigned ng = neg ^ ((q < 0.0f) ? 1u : 0u);
-        if (UPDATE_APP) {
-            const float mag = (e == idx) ? m2 : m1;
-            a[addr] = q + (ng ? -mag : mag);
+#pragma unroll
+    for (int g = 0; g < MAXT; g += G) {
+        if (g < tmax) {
+#pragma unroll
+            for (int t = g; t < g + G; ++t) {
+                const int e = t * P + p;
+                if (e < dc) {
+                    const unsigned ng = neg ^ ((q[t] < 0.0f) ? 1u : 0u);
+                    if (UPDATE_APP) {
+                        const float mag = (e == idx) ? m2 : m1;
+                        a[addr[t]] = q[t] + (ng ? -mag : mag);
+                    }
+                    signs |= ng << e;
+                }
+            }
         }
-        signs |= ng << e;
     }
 #pragma unroll
     for (int off = 1; off < P; off <<= 1)
         signs |= static_cast<unsigned>(__shfl_xor(static_cast<int>(signs), off));
-    if (p == 0) {
-        CheckState out;
-        out.m1 = m1;
-        out.m2 = m2;
-        out.signs = signs;
-        out.idx = idx;
-        *st_ptr = out;
-    }
+    CheckState out;
+    out.m1 = m1;
+    out.m2 = m2;
+    out.signs = signs;
+    out.idx = idx;
+    return out;
 }
 
-// One sum-product check row; Rm is the frame's messages, [slot][Z].
+// One sum-product check row; Rm is the frame's messages, [slot][Z].  The old
+// messages of the row are loaded together up front; Q, phi(|Q|) and the LDS
+// address of every edge stay in registers between the passes.
 template <bool UPDATE_APP>
 __device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, float* a,
                                           float* __restrict__ Rm, int br, int r, bool first) {
+    constexpr int MAXT = kMaxDegree;
     const int Z = c.lifting;
-    const int s0 = c.qc_row_ptr[br];
-    const int s1 = c.qc_row_ptr[br + 1];
+    const int s0 = ld_uniform(c.qc_row_ptr, br);
+    const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
+    float rold[MAXT];
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t)
+        if (t < dc) rold[t] = first ? 0.0f : Rm[static_cast<size_t>(s0 + t) * Z + r];
+    float q[MAXT];
+    float ph[MAXT];
+    int addr[MAXT];
     float total = 0.0f;
     unsigned neg = 0u;
-    for (int s = s0; s < s1; ++s) {
-        const float rold = first ? 0.0f : Rm[static_cast<size_t>(s) * Z + r];
-        const float q = a[qc_col(c.qc_slot_col[s], r, c.qc_slot_shift[s], Z)] - rold;
-        neg ^= (q < 0.0f) ? 1u : 0u;
-        total = total + phi(fabsf(q));
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t) {
+        if (t < dc) {
+            int jz[1], sh[1];
+            slot_group<1, 1>(c, s0 + t, 0, jz, sh);
+            addr[t] = qc_addr(jz[0], r, sh[0], Z);
+            q[t] = a[addr[t]] - rold[t];
+            neg ^= (q[t] < 0.0f) ? 1u : 0u;
+            ph[t] = phi(fabsf(q[t]));
+            total = total + ph[t];
+        }
     }
-    for (int s = s0; s < s1; ++s) {
-        float* rp = Rm + static_cast<size_t>(s) * Z + r;
-        const float rold = first ? 0.0f : *rp;
-        const int addr = qc_col(c.qc_slot_col[s], r, c.qc_slot_shift[s], Z);
-        const float q = a[addr] - rold;
-        const float mag = phi(total - phi(fabsf(q)));
-        const float rn = (neg ^ ((q < 0.0f) ? 1u : 0u)) ? -mag : mag;
-        if (UPDATE_APP) a[addr] = q + rn;
-        *rp = rn;
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t) {
+        if (t < dc) {
+            const float mag = phi(total - ph[t]);
+            const float rn = (neg ^ ((q[t] < 0.0f) ? 1u : 0u)) ? -mag : mag;
+            if (UPDATE_APP) a[addr[t]] = q[t] + rn;
+            Rm[static_cast<size_t>(s0 + t) * Z + r] = rn;
+        }
     }
 }
 
@@ -302,6 +389,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
     __shared__ int s_flag[kMaxFrames];
     const int N = c.num_variables;
     const int Z = c.lifting;
+    const int mb = c.base_rows;
     const QcLane l = qc_lane<P>(Z, F, R);
     float* a = lds + static_cast<size_t>(l.f < F ? l.f : 0) * N;
     load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
@@ -327,15 +415,26 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
         done = done || ok;
     }
     int it = 0;
+    // The state of the next block row is loaded while the current one is
+    // processed; before the first iteration every state is zero.
+    CheckState st_next = zero_state();
     while (__syncthreads_or(!done) && it < prm.max_iterations) {
         const bool first = it == 0;
-        for (int br = 0; br < c.base_rows; ++br) {
+        for (int br = 0; br < mb; ++br) {
             if (!done) {
                 if (SP) {
                     qc_sp_row<true>(c, a, Rm, br, l.r, first);
                 } else {
-                    qc_ms_row<P, true>(c, a, st_base + static_cast<size_t>(br) * Z + l.r, br,
-                                       l.r, l.p, first, prm);
+                    const CheckState st = st_next;
+                    if (br + 1 < mb) {
+                        st_next = first ? zero_state()
+                                        : st_base[static_cast<size_t>(br + 1) * Z + l.r];
+                    } else if (mb > 1) {
+                        st_next = st_base[l.r];  // block row 0, stored this iteration
+                    }
+                    const CheckState ns = qc_ms_row<P, true>(c, a, st, br, l.r, l.p, prm);
+                    if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
+                    if (mb == 1) st_next = ns;
                 }
             }
             __syncthreads();
@@ -391,8 +490,10 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
                 if (SP) {
                     if (l.p == 0) qc_sp_row<false>(c, a, Rm, br, l.r, first);
                 } else {
-                    qc_ms_row<P, false>(c, a, st_base + static_cast<size_t>(br) * Z + l.r, br,
-                                        l.r, l.p, first, prm);
+                    CheckState* sp = st_base + static_cast<size_t>(br) * Z + l.r;
+                    const CheckState st = first ? zero_state() : *sp;
+                    const CheckState ns = qc_ms_row<P, false>(c, a, st, br, l.r, l.p, prm);
+                    if (l.p == 0) *sp = ns;
                 }
             }
         }
@@ -584,7 +685,8 @@ int32_t check_code(const ck_ldpc_code* c) {
                    c->base_cols, c->lifting, c->num_checks, c->num_variables);
         CK_REQUIRE(c->num_slots * c->lifting == c->num_edges, "QC slots do not cover the edges");
         CK_REQUIRE(c->qc_row_ptr && c->qc_slot_col && c->qc_slot_shift && c->qc_row_id &&
-                       c->qc_col_ptr && c->qc_col_slot && c->qc_slot_pos && c->qc_slot_row,
+                       c->qc_col_ptr && c->qc_col_slot && c->qc_slot_pos && c->qc_slot_row &&
+                       c->qc_slot_packed,
                    "NULL QC table");
     }
     return CK_OK;

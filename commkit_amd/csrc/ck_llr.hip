@@ -32,7 +32,6 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr float kLogFloor = -59.867214f;  // logf(1e-26f)
 
 __device__ __forceinline__ int tile_stride(int k) { return k | 1; }
 
@@ -62,12 +61,15 @@ __device__ __forceinline__ void store_tile(const float* tile, float* __restrict_
     }
 }
 
-// LDS copy of one axis with 2^KB levels.
+// LDS copy of one axis with 2^KB levels: float64 levels and log prior for
+// the metric differences, float32 copies for locating maxima.
 template <int KB>
 struct AxisTab {
     static constexpr int L = 1 << KB;
-    float lev[L];
-    float logp[L];
+    double lev[L];
+    double logp[L];
+    float levf[L];
+    float logpf[L];
     int lab[L];
     short lo[KB][L];
     short hi[KB][L];
@@ -80,6 +82,8 @@ __device__ void stage_axis(AxisTab<KB>& t, const ck_axis& a, bool with_opposite)
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
         t.lev[i] = a.levels[i];
         t.logp[i] = a.log_prior[i];
+        t.levf[i] = static_cast<float>(a.levels[i]);
+        t.logpf[i] = static_cast<float>(a.log_prior[i]);
         t.lab[i] = a.labels[i];
     }
     if (with_opposite) {
@@ -91,34 +95,37 @@ __device__ void stage_axis(AxisTab<KB>& t, const ck_axis& a, bool with_opposite)
     if (threadIdx.x < KB) t.pos[threadIdx.x] = a.pos[threadIdx.x];
 }
 
-// m_a - m_c of two levels of one axis, factored (see the file comment).
+// m_a - m_c of two levels of one axis, in float64, factored (see the file
+// comment): exact up to float64 rounding for the float32 symbol x.
 template <int KB>
-__device__ __forceinline__ float level_diff(const AxisTab<KB>& t, float x, float inv_s2,
-                                            int a, int c) {
-    const float la = t.lev[a];
-    const float lc = t.lev[c];
+__device__ __forceinline__ double level_diff(const AxisTab<KB>& t, double x, double inv_s2,
+                                             int a, int c) {
+    const double la = t.lev[a];
+    const double lc = t.lev[c];
     return (la - lc) * ((x - la) + (x - lc)) * inv_s2 + (t.logp[a] - t.logp[c]);
 }
 
-// Closed-form max-log LLRs of one axis into row[pos[b]].
+// Closed-form max-log LLRs of one axis into row[pos[b]]: the nearest level
+// against the nearer of the two levels just outside its run of equal bits.
 template <int KB>
-__device__ __forceinline__ void axis_closed(const AxisTab<KB>& t, float x, float inv_s2,
+__device__ __forceinline__ void axis_closed(const AxisTab<KB>& t, float x, double inv_s2,
                                             float inv_step, float* row) {
     constexpr int L = 1 << KB;
-    int s = __float2int_rn((x - t.lev[0]) * inv_step);
+    int s = __float2int_rn((x - t.levf[0]) * inv_step);
     s = min(max(s, 0), L - 1);
-    float d = x - t.lev[s];
-    // The rounded quotient can land one level off near a midpoint: keep the
-    // nearer of s and its neighbour towards x.
-    const int s2 = d > 0.0f ? s + 1 : s - 1;
+    const double xd = x;
+    double d = xd - t.lev[s];
+    // The float32 estimate can land one level off near a midpoint: keep the
+    // nearer of s and its neighbour towards x, compared in float64.
+    const int s2 = d > 0.0 ? s + 1 : s - 1;
     if (s2 >= 0 && s2 < L) {
-        const float d2 = x - t.lev[s2];
-        if (d2 * d2 < d * d) {
+        const double d2 = xd - t.lev[s2];
+        if (fabs(d2) < fabs(d)) {
             s = s2;
             d = d2;
         }
     }
-    const float ls = t.lev[s];
+    const double ls = t.lev[s];
     const int lab = t.lab[s];
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
@@ -130,25 +137,65 @@ __device__ __forceinline__ void axis_closed(const AxisTab<KB>& t, float x, float
         } else if (jh < 0) {
             j = jl;
         } else {
-            const float dl = x - t.lev[jl];
-            const float dh = x - t.lev[jh];
-            j = (dh * dh < dl * dl) ? jh : jl;
+            j = (fabs(xd - t.lev[jh]) < fabs(xd - t.lev[jl])) ? jh : jl;
         }
         j = j < 0 ? s : j;  // a bit constant over the axis: no opposite level
-        const float lo = t.lev[j];
-        // m_own - m_other, the LLR when the snapped level's bit is 0.
-        const float diff = (ls - lo) * (d + (x - lo)) * inv_s2;
-        row[t.pos[b]] = ((lab >> (KB - 1 - b)) & 1) ? -diff : diff;
+        const double lo = t.lev[j];
+        // m_own - m_other, the LLR when the nearest level's bit is 0.
+        const double diff = (ls - lo) * (d + (xd - lo)) * inv_s2;
+        const float v = static_cast<float>(diff);
+        row[t.pos[b]] = ((lab >> (KB - 1 - b)) & 1) ? -v : v;
     }
 }
 
+// Below this (in nats) a half's maximum is too far under the peak for its
+// exp to stay a normal float32; that half is summed relative to itself.
+constexpr double kWeak = -80.0;
+constexpr float kLog2e = 1.4426950408889634f;
+
 // LLRs of one axis from the metrics of its levels into row[pos[b]].
+// Max-log: the difference of the two halves' maxima, located in float64.
+// Exact: LLR = (m_a0 - m_a1) + log(1 + T0) - log(1 + T1), with a_h the
+// maximum of half h and T_h = sum over the other levels of half h of
+// exp(m_l - m_a_h); the exponentials are taken once per level relative to
+// the symbol's peak and divided by the half's own term (or, for a half
+// whose maximum is more than 80 nats under the peak, taken relative to that
+// maximum directly).
 template <int KB>
-__device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float inv_s2,
-                                            bool exact, float* row) {
+__device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float inv_s2f,
+                                            double inv_s2, bool exact, float* row) {
     constexpr int L = 1 << KB;
-    float p0[KB], p1[KB];
+    const double xd = x;
     int a0[KB], a1[KB];
+    if (!exact) {
+        double p0[KB], p1[KB];
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            p0[b] = -1.0e300;
+            p1[b] = -1.0e300;
+            a0[b] = 0;
+            a1[b] = 0;
+        }
+#pragma unroll 4
+        for (int l = 0; l < L; ++l) {
+            const double dx = xd - t.lev[l];
+            const double m = -(dx * dx) * inv_s2 + t.logp[l];
+            const int lab = t.lab[l];
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                if ((lab >> (KB - 1 - b)) & 1) {
+                    if (m > p1[b]) { p1[b] = m; a1[b] = l; }
+                } else {
+                    if (m > p0[b]) { p0[b] = m; a0[b] = l; }
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < KB; ++b)
+            row[t.pos[b]] = static_cast<float>(level_diff(t, xd, inv_s2, a0[b], a1[b]));
+        return;
+    }
+    float p0[KB], p1[KB];
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
         p0[b] = -ck::kInf;
@@ -158,8 +205,8 @@ __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float
     }
 #pragma unroll 4
     for (int l = 0; l < L; ++l) {
-        const float dx = x - t.lev[l];
-        const float m = -(dx * dx) * inv_s2 + t.logp[l];
+        const float dx = x - t.levf[l];
+        const float m = -(dx * dx) * inv_s2f + t.logpf[l];
         const int lab = t.lab[l];
 #pragma unroll
         for (int b = 0; b < KB; ++b) {
@@ -170,60 +217,66 @@ __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float
             }
         }
     }
-    if (!exact) {
-#pragma unroll
-        for (int b = 0; b < KB; ++b) row[t.pos[b]] = level_diff(t, x, inv_s2, a0[b], a1[b]);
-        return;
-    }
-    // Every level lies in one half of bit 0: the better of its two maxima is
-    // the peak.  A half whose own maximum lies below the peak by more than
-    // -log(1e-26) is summed relative to that maximum instead.
     const int pk = (p0[0] >= p1[0]) ? a0[0] : a1[0];
-    float w0[KB], w1[KB], s0[KB], s1[KB];
-    unsigned weak0 = 0u, weak1 = 0u;
+    double w0[KB], w1[KB];
+    float u0[KB], u1[KB];
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
-        w0[b] = level_diff(t, x, inv_s2, a0[b], pk);
-        w1[b] = level_diff(t, x, inv_s2, a1[b], pk);
-        weak0 |= (w0[b] < kLogFloor ? 1u : 0u) << b;
-        weak1 |= (w1[b] < kLogFloor ? 1u : 0u) << b;
-        s0[b] = 0.0f;
-        s1[b] = 0.0f;
+        w0[b] = level_diff(t, xd, inv_s2, a0[b], pk);
+        w1[b] = level_diff(t, xd, inv_s2, a1[b], pk);
+        u0[b] = 0.0f;
+        u1[b] = 0.0f;
     }
 #pragma unroll 4
     for (int l = 0; l < L; ++l) {
-        const float e = expf(level_diff(t, x, inv_s2, l, pk));
+        const float e = exp2f(static_cast<float>(level_diff(t, xd, inv_s2, l, pk)) * kLog2e);
         const int lab = t.lab[l];
 #pragma unroll
         for (int b = 0; b < KB; ++b) {
             if ((lab >> (KB - 1 - b)) & 1) {
-                s1[b] += ((weak1 >> b) & 1u) ? expf(level_diff(t, x, inv_s2, l, a1[b])) : e;
+                u1[b] += (l == a1[b]) ? 0.0f : e;
             } else {
-                s0[b] += ((weak0 >> b) & 1u) ? expf(level_diff(t, x, inv_s2, l, a0[b])) : e;
+                u0[b] += (l == a0[b]) ? 0.0f : e;
             }
         }
     }
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
-        const float l0 = logf(s0[b]) + (((weak0 >> b) & 1u) ? w0[b] : 0.0f);
-        const float l1 = logf(s1[b]) + (((weak1 >> b) & 1u) ? w1[b] : 0.0f);
-        row[t.pos[b]] = l0 - l1;
+        float t0, t1;
+        if (w0[b] < kWeak) {
+            t0 = 0.0f;
+            for (int l = 0; l < L; ++l)
+                if (!((t.lab[l] >> (KB - 1 - b)) & 1) && l != a0[b])
+                    t0 += exp2f(static_cast<float>(level_diff(t, xd, inv_s2, l, a0[b])) * kLog2e);
+        } else {
+            t0 = u0[b] / exp2f(static_cast<float>(w0[b]) * kLog2e);
+        }
+        if (w1[b] < kWeak) {
+            t1 = 0.0f;
+            for (int l = 0; l < L; ++l)
+                if (((t.lab[l] >> (KB - 1 - b)) & 1) && l != a1[b])
+                    t1 += exp2f(static_cast<float>(level_diff(t, xd, inv_s2, l, a1[b])) * kLog2e);
+        } else {
+            t1 = u1[b] / exp2f(static_cast<float>(w1[b]) * kLog2e);
+        }
+        const double d01 = level_diff(t, xd, inv_s2, a0[b], a1[b]);
+        row[t.pos[b]] = static_cast<float>(d01 + static_cast<double>(log1pf(t0) - log1pf(t1)));
     }
 }
 
 // CLOSED: closed form (max-log, uniform); else level metrics.
 template <int KB, bool CLOSED>
 __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
-    const float2* __restrict__ x, long long n, ck_axis re, ck_axis im, float inv_s2,
-    int exact, float* __restrict__ out) {
+    const float2* __restrict__ x, long long n, ck_axis re, ck_axis im, float inv_s2f,
+    double inv_s2, int exact, float* __restrict__ out) {
     constexpr int K = 2 * KB;
     __shared__ AxisTab<KB> tab[2];
     __shared__ float tile[kThreads * (K | 1)];
     stage_axis<KB>(tab[0], re, CLOSED);
     stage_axis<KB>(tab[1], im, CLOSED);
     __syncthreads();
-    const float inv_step0 = 1.0f / (tab[0].lev[1] - tab[0].lev[0]);
-    const float inv_step1 = 1.0f / (tab[1].lev[1] - tab[1].lev[0]);
+    const float inv_step0 = 1.0f / (tab[0].levf[1] - tab[0].levf[0]);
+    const float inv_step1 = 1.0f / (tab[1].levf[1] - tab[1].levf[0]);
     float* row = tile + threadIdx.x * tile_stride(K);
     for (long long base = static_cast<long long>(blockIdx.x) * kThreads; base < n;
          base += static_cast<long long>(gridDim.x) * kThreads) {
@@ -234,8 +287,8 @@ __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
                 axis_closed<KB>(tab[0], v.x, inv_s2, inv_step0, row);
                 axis_closed<KB>(tab[1], v.y, inv_s2, inv_step1, row);
             } else {
-                axis_levels<KB>(tab[0], v.x, inv_s2, exact != 0, row);
-                axis_levels<KB>(tab[1], v.y, inv_s2, exact != 0, row);
+                axis_levels<KB>(tab[0], v.x, inv_s2f, inv_s2, exact != 0, row);
+                axis_levels<KB>(tab[1], v.y, inv_s2f, inv_s2, exact != 0, row);
             }
         }
         __syncthreads();
@@ -245,32 +298,32 @@ __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
     }
 }
 
-// m_a - m_c of two points, factored per component.
-__device__ __forceinline__ float point_diff(const float2* pts, const float* logp, float xr,
-                                            float xi, float inv_s2, int a, int c) {
-    const float2 pa = pts[a];
-    const float2 pc = pts[c];
-    const float dr = (pa.x - pc.x) * ((xr - pa.x) + (xr - pc.x));
-    const float di = (pa.y - pc.y) * ((xi - pa.y) + (xi - pc.y));
+// m_a - m_c of two points in float64, factored per component.
+__device__ __forceinline__ double point_diff(const double2* pts, const double* logp, double xr,
+                                             double xi, double inv_s2, int a, int c) {
+    const double2 pa = pts[a];
+    const double2 pc = pts[c];
+    const double dr = (pa.x - pc.x) * ((xr - pa.x) + (xr - pc.x));
+    const double di = (pa.y - pc.y) * ((xi - pa.y) + (xi - pc.y));
     return (dr + di) * inv_s2 + (logp[a] - logp[c]);
 }
 
-constexpr int kPointsInLds = 2048;
+constexpr int kPointsInLds = 1024;
 
 template <int K>
 __global__ __launch_bounds__(kThreads) void llr_points_kernel(
-    const float2* __restrict__ x, long long n, const float2* __restrict__ g_pts,
-    const float* __restrict__ g_logp, const int* __restrict__ g_lab, int M, float inv_s2,
-    int exact, float* __restrict__ out) {
+    const float2* __restrict__ x, long long n, const double2* __restrict__ g_pts,
+    const double* __restrict__ g_logp, const int* __restrict__ g_lab, int M, float inv_s2f,
+    double inv_s2, int exact, float* __restrict__ out) {
     __shared__ float tile[kThreads * (K | 1)];
     extern __shared__ char dyn[];
     const bool staged = M <= kPointsInLds;
-    const float2* pts = g_pts;
-    const float* logp = g_logp;
+    const double2* pts = g_pts;
+    const double* logp = g_logp;
     const int* lab_tab = g_lab;
     if (staged) {
-        float2* s_pts = reinterpret_cast<float2*>(dyn);
-        float* s_logp = reinterpret_cast<float*>(s_pts + M);
+        double2* s_pts = reinterpret_cast<double2*>(dyn);
+        double* s_logp = reinterpret_cast<double*>(s_pts + M);
         int* s_lab = reinterpret_cast<int*>(s_logp + M);
         for (int i = threadIdx.x; i < M; i += kThreads) {
             s_pts[i] = g_pts[i];
@@ -288,68 +341,108 @@ __global__ __launch_bounds__(kThreads) void llr_points_kernel(
         const long long i = base + threadIdx.x;
         if (i < n) {
             const float2 v = x[i];
-            float p0[K], p1[K];
+            const double xr = v.x;
+            const double xi = v.y;
             int a0[K], a1[K];
-#pragma unroll
-            for (int b = 0; b < K; ++b) {
-                p0[b] = -ck::kInf;
-                p1[b] = -ck::kInf;
-                a0[b] = 0;
-                a1[b] = 0;
-            }
-            for (int m = 0; m < M; ++m) {
-                const float2 p = pts[m];
-                const float dr = v.x - p.x;
-                const float di = v.y - p.y;
-                const float met = -(dr * dr + di * di) * inv_s2 + logp[m];
-                const int lab = lab_tab[m];
-#pragma unroll
-                for (int b = 0; b < K; ++b) {
-                    if ((lab >> (K - 1 - b)) & 1) {
-                        if (met > p1[b]) { p1[b] = met; a1[b] = m; }
-                    } else {
-                        if (met > p0[b]) { p0[b] = met; a0[b] = m; }
-                    }
-                }
-            }
+            // Maxima of each half: located in float64 for max-log (they are
+            // the answer), in float32 for exact (only a reference point).
             if (!exact) {
-#pragma unroll
-                for (int b = 0; b < K; ++b)
-                    row[b] = point_diff(pts, logp, v.x, v.y, inv_s2, a0[b], a1[b]);
-            } else {
-                const int pk = (p0[0] >= p1[0]) ? a0[0] : a1[0];
-                float w0[K], w1[K], s0[K], s1[K];
-                unsigned weak0 = 0u, weak1 = 0u;
+                double p0[K], p1[K];
 #pragma unroll
                 for (int b = 0; b < K; ++b) {
-                    w0[b] = point_diff(pts, logp, v.x, v.y, inv_s2, a0[b], pk);
-                    w1[b] = point_diff(pts, logp, v.x, v.y, inv_s2, a1[b], pk);
-                    weak0 |= (w0[b] < kLogFloor ? 1u : 0u) << b;
-                    weak1 |= (w1[b] < kLogFloor ? 1u : 0u) << b;
-                    s0[b] = 0.0f;
-                    s1[b] = 0.0f;
+                    p0[b] = -1.0e300;
+                    p1[b] = -1.0e300;
+                    a0[b] = 0;
+                    a1[b] = 0;
                 }
                 for (int m = 0; m < M; ++m) {
-                    const float e = expf(point_diff(pts, logp, v.x, v.y, inv_s2, m, pk));
+                    const double2 p = pts[m];
+                    const double dr = xr - p.x;
+                    const double di = xi - p.y;
+                    const double met = -(dr * dr + di * di) * inv_s2 + logp[m];
                     const int lab = lab_tab[m];
 #pragma unroll
                     for (int b = 0; b < K; ++b) {
                         if ((lab >> (K - 1 - b)) & 1) {
-                            s1[b] += ((weak1 >> b) & 1u)
-                                         ? expf(point_diff(pts, logp, v.x, v.y, inv_s2, m, a1[b]))
-                                         : e;
+                            if (met > p1[b]) { p1[b] = met; a1[b] = m; }
                         } else {
-                            s0[b] += ((weak0 >> b) & 1u)
-                                         ? expf(point_diff(pts, logp, v.x, v.y, inv_s2, m, a0[b]))
-                                         : e;
+                            if (met > p0[b]) { p0[b] = met; a0[b] = m; }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < K; ++b)
+                    row[b] = static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2, a0[b], a1[b]));
+            } else {
+                float p0[K], p1[K];
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    p0[b] = -ck::kInf;
+                    p1[b] = -ck::kInf;
+                    a0[b] = 0;
+                    a1[b] = 0;
+                }
+                for (int m = 0; m < M; ++m) {
+                    const double2 p = pts[m];
+                    const float dr = v.x - static_cast<float>(p.x);
+                    const float di = v.y - static_cast<float>(p.y);
+                    const float met = -(dr * dr + di * di) * inv_s2f + static_cast<float>(logp[m]);
+                    const int lab = lab_tab[m];
+#pragma unroll
+                    for (int b = 0; b < K; ++b) {
+                        if ((lab >> (K - 1 - b)) & 1) {
+                            if (met > p1[b]) { p1[b] = met; a1[b] = m; }
+                        } else {
+                            if (met > p0[b]) { p0[b] = met; a0[b] = m; }
+                        }
+                    }
+                }
+                const int pk = (p0[0] >= p1[0]) ? a0[0] : a1[0];
+                double w0[K], w1[K];
+                float u0[K], u1[K];
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    w0[b] = point_diff(pts, logp, xr, xi, inv_s2, a0[b], pk);
+                    w1[b] = point_diff(pts, logp, xr, xi, inv_s2, a1[b], pk);
+                    u0[b] = 0.0f;
+                    u1[b] = 0.0f;
+                }
+                for (int m = 0; m < M; ++m) {
+                    const float e = exp2f(
+                        static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2, m, pk)) * kLog2e);
+                    const int lab = lab_tab[m];
+#pragma unroll
+                    for (int b = 0; b < K; ++b) {
+                        if ((lab >> (K - 1 - b)) & 1) {
+                            u1[b] += (m == a1[b]) ? 0.0f : e;
+                        } else {
+                            u0[b] += (m == a0[b]) ? 0.0f : e;
                         }
                     }
                 }
 #pragma unroll
                 for (int b = 0; b < K; ++b) {
-                    const float l0 = logf(s0[b]) + (((weak0 >> b) & 1u) ? w0[b] : 0.0f);
-                    const float l1 = logf(s1[b]) + (((weak1 >> b) & 1u) ? w1[b] : 0.0f);
-                    row[b] = l0 - l1;
+                    float t0, t1;
+                    if (w0[b] < kWeak) {
+                        t0 = 0.0f;
+                        for (int m = 0; m < M; ++m)
+                            if (!((lab_tab[m] >> (K - 1 - b)) & 1) && m != a0[b])
+                                t0 += exp2f(static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2,
+                                                                          m, a0[b])) * kLog2e);
+                    } else {
+                        t0 = u0[b] / exp2f(static_cast<float>(w0[b]) * kLog2e);
+                    }
+                    if (w1[b] < kWeak) {
+                        t1 = 0.0f;
+                        for (int m = 0; m < M; ++m)
+                            if (((lab_tab[m] >> (K - 1 - b)) & 1) && m != a1[b])
+                                t1 += exp2f(static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2,
+                                                                          m, a1[b])) * kLog2e);
+                    } else {
+                        t1 = u1[b] / exp2f(static_cast<float>(w1[b]) * kLog2e);
+                    }
+                    const double d01 = point_diff(pts, logp, xr, xi, inv_s2, a0[b], a1[b]);
+                    row[b] = static_cast<float>(d01 + static_cast<double>(log1pf(t0) - log1pf(t1)));
                 }
             }
         }
@@ -398,7 +491,8 @@ int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_a
 #define CK_CASE(KB)                                                                           \
     case KB:                                                                                  \
         hipLaunchKernelGGL((llr_separable_kernel<KB, CLOSED>), dim3(grid), dim3(kThreads), 0, \
-                           s, xs, static_cast<long long>(n), *re, *im, inv_s2, exact, out);   \
+                           s, xs, static_cast<long long>(n), *re, *im, inv_s2,                \
+                           static_cast<double>(inv_s2), exact, out);                          \
         break;
         CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
 #undef CK_CASE
@@ -422,7 +516,7 @@ extern "C" int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
 }
 
 extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,
-                                 const float* log_prior, const int32_t* labels, int32_t M,
+                                 const double* log_prior, const int32_t* labels, int32_t M,
                                  int32_t k, float inv_s2, int32_t exact, float* out,
                                  void* stream) {
     CK_REQUIRE(n >= 0, "n = %lld < 0", static_cast<long long>(n));
@@ -433,16 +527,16 @@ extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,
     if (n == 0) return CK_OK;
     CK_REQUIRE(x && out && points && log_prior && labels, "NULL pointer argument");
     const int grid = grid_for(n);
-    const size_t dyn = M <= kPointsInLds ? static_cast<size_t>(M) * 16 : 0;
+    const size_t dyn = M <= kPointsInLds ? static_cast<size_t>(M) * 28 : 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const float2* xs = static_cast<const float2*>(x);
-    const float2* pts = static_cast<const float2*>(points);
+    const double2* pts = static_cast<const double2*>(points);
     switch (k) {
 #define CK_CASE(K)                                                                           \
     case K:                                                                                  \
         hipLaunchKernelGGL((llr_points_kernel<K>), dim3(grid), dim3(kThreads), dyn, s, xs,   \
                            static_cast<long long>(n), pts, log_prior, labels, M, inv_s2,     \
-                           exact, out);                                                      \
+                           static_cast<double>(inv_s2), exact, out);                         \
         break;
         CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
         CK_CASE(9) CK_CASE(10) CK_CASE(11) CK_CASE(12) CK_CASE(13) CK_CASE(14) CK_CASE(15)

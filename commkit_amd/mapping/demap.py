@@ -116,11 +116,18 @@ class CkAxis(ctypes.Structure):
     ]
 
 
+def log_prior64(pmf: np.ndarray | None, order: int) -> np.ndarray:
+    """float64 log prior, ``log(clip(pmf, 1e-40))``; zeros for uniform."""
+    if pmf is None:
+        return np.zeros(order, dtype=np.float64)
+    return np.log(np.clip(np.asarray(pmf, np.float64), 1e-40, None))
+
+
 def _axis_tables(a: Axis) -> tuple[np.ndarray, ...]:
     lo, hi = opposite_levels(a.labels)
     return (
-        a.levels.astype(np.float32),
-        log_prior(a.pmf, a.levels.size),
+        a.levels.astype(np.float64),
+        log_prior64(a.pmf, a.levels.size),
         pack_bits(a.labels).astype(np.int32),
         lo,
         hi,
@@ -188,8 +195,8 @@ def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
                 pts, lp, lab = device_tables(
                     c, "points", x.device,
                     lambda: (
-                        c.points.astype(np.complex64),
-                        log_prior(c.pmf, c.order),
+                        c.points.astype(np.complex128),
+                        log_prior64(c.pmf, c.order),
                         pack_bits(c.bit_labels).astype(np.int32),
                     ),
                 )  # fmt: skip

@@ -26,8 +26,12 @@ from typing import Any
 
 __all__ = ["WATERFALL_DB", "waterfall_db", "calibrate", "frame_error_rate"]
 
-#: Measured waterfalls (dB) by (code name, constellation order).
-WATERFALL_DB: dict[tuple[str, int], float] = {}
+#: Measured waterfalls (dB) by (code name, constellation order), from
+#: ``python -m commkit_amd.operating`` on an MI355X.
+WATERFALL_DB: dict[tuple[str, int], float] = {
+    ("r4/5-z360", 64): 16.65,
+    ("r1/2-z27", 16): 9.2,
+}
 
 _GRID = 0.05
 _FRAMES = 1024

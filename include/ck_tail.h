@@ -45,10 +45,12 @@ const char* ck_last_error(void);
 /* One axis of a separable square lattice (Gray QAM, shaped or not); all
  * pointers are device memory.  Reference: the per-axis tables built by
  * commkit/mapping/_demap.py:411-441 (levels, log prior, packed labels,
- * opposite levels, bit positions). */
+ * opposite levels, bit positions).  Levels and prior are float64 here (the
+ * reference uploads float32): the kernels form metric differences in
+ * float64 from them, so an LLR keeps its relative accuracy down to zero. */
 typedef struct ck_axis {
-    const float* levels;     /* (L,) ascending, equally spaced for max-log */
-    const float* log_prior;  /* (L,) log marginal prior; zeros = uniform   */
+    const double* levels;    /* (L,) ascending, equally spaced for max-log */
+    const double* log_prior; /* (L,) log marginal prior; zeros = uniform   */
     const int32_t* labels;   /* (L,) label of each level over the axis's
                                 num_bits bits, MSB first                  */
     const int32_t* lo;       /* (num_bits, L) nearest level below whose bit
@@ -79,13 +81,13 @@ int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
                          float* out, void* stream);
 
 /* LLRs from the metrics of all M points (any constellation).
- * points: (M,) complex64 device; log_prior: (M,) float32 device (zeros =
+ * points: (M,) complex128 device; log_prior: (M,) float64 device (zeros =
  * uniform); labels: (M,) int32 packed MSB-first labels; 2 <= M <= 65536,
  * k = log2(M) <= 16.  out: (n, k) float32.
  * Replaces `llr` (commkit/_cuda/src/llr.cu:200-223; wrapper
  * commkit/_cuda/llr.py; call site commkit/mapping/_demap.py:305). */
 int32_t ck_llr_points(const void* x, int64_t n, const void* points,
-                      const float* log_prior, const int32_t* labels, int32_t M,
+                      const double* log_prior, const int32_t* labels, int32_t M,
                       int32_t k, float inv_s2, int32_t exact, float* out,
                       void* stream);
 
@@ -128,6 +130,8 @@ typedef struct ck_ldpc_code {
     const int32_t* qc_col_slot;   /* (S) ... in ascending natural block row  */
     const int32_t* qc_slot_pos;   /* (S) position of the slot in its row     */
     const int32_t* qc_slot_row;   /* (S) scheduled block row of each slot    */
+    const int32_t* qc_slot_packed;/* (S + 32, 2) {block column * Z, shift}
+                                     per slot, then 32 zero entries         */
 } ck_ldpc_code;
 
 /* Decoding method.  rule/scale/offset/max_iterations/early_stop follow

@@ -40,30 +40,11 @@ def llr_fp64(x: np.ndarray, points: np.ndarray, bit_labels: np.ndarray, pmf: np.
     return out.reshape(-1)
 
 
-def error_scale(x: np.ndarray, points: np.ndarray, noise_var: float) -> np.ndarray:
-    """Per-LLR size of the float32 input rounding, ``(N * k,)``.
-
-    A float32 evaluation sees the levels and the symbol rounded to float32
-    (relative 2^-24); a metric difference ``(|x - a|^2 - |x - c|^2) / sigma^2``
-    then moves by up to about ``4 eps |x| d_max / sigma^2``, with ``d_max`` the
-    constellation's diameter.  LLRs are compared within
-    ``1e-5 * max(|L64|, 2^10 * this)``: relative 1e-5 except where the LLR is
-    itself within a few thousand roundings of zero.
-    """
-    x = np.asarray(x).ravel()
-    pts = np.asarray(points, dtype=np.complex128)
-    reach = np.abs(x).astype(np.float64) + np.abs(pts).max()
-    d_max = np.abs(pts[:, None] - pts[None, :]).max()
-    s = 4.0 * 2.0**-24 * reach * d_max / noise_var
-    return s
-
-
-def llr_tolerance_ok(got: np.ndarray, ref: np.ndarray, scale: np.ndarray, k: int,
-                     rel: float = 1e-5) -> tuple[bool, np.ndarray]:
-    """Whether ``|got - ref| <= rel * max(|ref|, 2^10 * scale)``; also the mask of
-    elements inside the floor (where pure relative ``rel`` is not claimed)."""
-    scale_k = np.repeat(scale, k)
-    floor = 1024.0 * scale_k
+def llr_tolerance_ok(got: np.ndarray, ref: np.ndarray, rel: float = 1e-5,
+                     floor: float = 0.0) -> tuple[bool, np.ndarray]:
+    """Whether ``|got - ref| <= rel * max(|ref|, floor)`` everywhere; also the
+    mask of elements under the floor (where pure relative ``rel`` is not
+    claimed)."""
     bound = rel * np.maximum(np.abs(ref), floor)
     ok = np.abs(got.astype(np.float64) - ref) <= bound
     return bool(ok.all()), np.abs(ref) < floor

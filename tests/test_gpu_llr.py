@@ -1,13 +1,19 @@
 """HIP demapper against the FP64 definition and the float32 CPU oracle (GPU).
 
-Tolerance (stated here, as the LLR contract): an LLR L32 from the GPU meets
-``|L32 - L64| <= 1e-5 * max(|L64|, floor)`` with L64 the float64 LLR of the
-definition, and ``floor = 2^10 * s`` where ``s`` bounds what float32 input
-rounding alone does to a metric difference (``oracle.llr64.error_scale``).
-So LLRs are within relative 1e-5 except where they are themselves within a
-few thousand roundings of zero; each case prints how many fell under the
-floor.  Where |L64| exceeds the floor, the sign (the hard decision) must
-agree.
+Tolerance (the LLR contract, stated here): with L64 the float64 LLR of the
+definition (float64 constellation, float64 arithmetic, the same complex64
+symbols), a GPU LLR L32 meets
+
+- max-log: ``|L32 - L64| <= 1e-5 * |L64|`` on every element, with no floor
+  (the kernels form metric differences in float64 from float64 tables, so
+  only the float32 ``1/sigma^2`` and the float32 output round);
+- exact (log-sum-exp): ``|L32 - L64| <= 1e-5 * max(|L64|, 2^-5)``: relative
+  1e-5 except within 2^-5 of zero, where the float32 correction sums
+  ``log(1 + T)`` allow an absolute 3e-7.
+
+The sign (hard decision) must agree wherever |L64| exceeds the floor.  Each
+case prints its largest relative error and how many elements fell under
+the floor.
 """
 
 import numpy as np
@@ -17,7 +23,9 @@ from commkit_amd import Constellation, Signal, compute_llr
 from commkit_amd.mapping.demap import log_prior, opposite_levels
 from commkit_amd.mapping.constellation import pack_bits
 from oracle import cpu
-from oracle.llr64 import error_scale, llr_fp64, llr_tolerance_ok
+from oracle.llr64 import llr_fp64, llr_tolerance_ok
+
+FLOOR = {"maxlog": 0.0, "exact": 2.0**-5}
 
 pytestmark = pytest.mark.gpu
 
@@ -55,12 +63,12 @@ def test_llr_against_fp64(gpu, name, method):
     assert got.dtype == torch.float32 and got.shape == (x.size * c.bits_per_symbol,)
     got = got.cpu().numpy()
     ref = llr_fp64(x, c.points, c.bit_labels, c.pmf, nv, method)
-    scale = error_scale(x, c.points, nv)
-    ok, under = llr_tolerance_ok(got, ref, scale, c.bits_per_symbol)
+    ok, under = llr_tolerance_ok(got, ref, floor=FLOOR[method])
     rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
+    err_abs = np.abs(got - ref)[under].max() if under.any() else 0.0
     print(f"{name} {method}: max rel err above floor {rel[~under].max():.2e}, "
-          f"{int(under.sum())} of {ref.size} under the floor")  # fmt: skip
-    assert ok, f"max rel err {rel[~under].max():.3e}"
+          f"{int(under.sum())} of {ref.size} under the floor (max abs err {err_abs:.2e})")
+    assert ok, f"max rel err {rel[~under].max():.3e}, abs under floor {err_abs:.3e}"
     assert np.array_equal(np.sign(got[~under]), np.sign(ref[~under]))
 
 
@@ -133,5 +141,5 @@ def test_real_pam(gpu):
     nv = 0.04  # real noise variance
     got = compute_llr(x, noise_var=nv, constellation=c)
     ref = llr_fp64(x, c.points, c.bit_labels, None, nv, "maxlog")
-    ok, under = llr_tolerance_ok(got, ref, error_scale(x, c.points, 2 * nv), 2)
+    ok, _ = llr_tolerance_ok(got, ref)
     assert ok
