@@ -25,14 +25,15 @@ _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
 _f32 = ctypes.c_float
+_f64 = ctypes.c_double
 
 #: Exported symbol -> (restype, argtypes); mirrors include/ck_tail.h.
 SYMBOLS: dict[str, tuple[Any, list[Any]]] = {
     "ck_version": (_i32, []),
     "ck_last_error": (ctypes.c_char_p, []),
-    "ck_llr_separable_maxlog": (_i32, [_p, _i64, _p, _p, _f32, _p, _p]),
-    "ck_llr_separable": (_i32, [_p, _i64, _p, _p, _f32, _i32, _p, _p]),
-    "ck_llr_points": (_i32, [_p, _i64, _p, _p, _p, _i32, _i32, _f32, _i32, _p, _p]),
+    "ck_llr_separable_maxlog": (_i32, [_p, _i64, _p, _p, _f64, _p, _p]),
+    "ck_llr_separable": (_i32, [_p, _i64, _p, _p, _f64, _i32, _p, _p]),
+    "ck_llr_points": (_i32, [_p, _i64, _p, _p, _p, _i32, _i32, _f64, _i32, _p, _p]),
     "ck_ldpc_workspace_bytes": (_i64, [_p, _p, _i64]),
     "ck_ldpc_decode": (_i32, [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _p]),
 }

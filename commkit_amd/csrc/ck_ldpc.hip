@@ -51,6 +51,9 @@ constexpr int kGenericThreads = 256;
 constexpr int kMaxQcThreads = 1024;
 constexpr int kMaxFrames = 64;   // frames per QC workgroup
 constexpr int kMaxDegree = 32;   // check degree of the compressed min-sum state
+constexpr int kMaxSlots = 480;   // nonzero circulants of a QC code the QC kernels take
+constexpr int kTabEntries = kMaxSlots + kMaxDegree;  // LDS slot table incl. padding
+constexpr size_t kStaticLds = 8 * kTabEntries + 4 * kMaxFrames;  // static LDS of the QC kernels
 constexpr float kPhiMin = 1e-7f;
 constexpr float kPhiMax = 30.0f;
 
@@ -168,30 +171,12 @@ __device__ __forceinline__ T ld_uniform(const T* p, int i) {
     return ((const CK_CONSTANT T*)p)[i];
 }
 
-// {block column * Z, shift} of slots base + t*P + p, t = 0..G-1, for the lane
-// with row-lane index p.  The G*P entries are consecutive in the packed,
-// zero-padded slot table and read with uniform addresses (one wide scalar
-// load); each lane keeps its own.  Entries past a row's end are real table
-// entries or padding, so their addresses stay inside the frame's APP.
-template <int P, int G>
-__device__ __forceinline__ void slot_group(const ck_ldpc_code& c, int base, int p, int* jz,
-                                           int* sh) {
-    const CK_CONSTANT int* tab = (const CK_CONSTANT int*)c.qc_slot_packed;
-#pragma unroll
-    for (int t = 0; t < G; ++t) {
-        const int i0 = 2 * (base + t * P);
-        int vx = tab[i0];
-        int vy = tab[i0 + 1];
-#pragma unroll
-        for (int u = 1; u < P; ++u) {
-            const int wx = tab[i0 + 2 * u];
-            const int wy = tab[i0 + 2 * u + 1];
-            vx = (p == u) ? wx : vx;
-            vy = (p == u) ? wy : vy;
-        }
-        jz[t] = vx;
-        sh[t] = vy;
-    }
+// Copies the packed {block column * Z, shift} slot table (S + 32 entries,
+// zero padded) into LDS, where every row update reads it.
+__device__ void stage_slot_table(const ck_ldpc_code& c, int2* tab) {
+    const int n = c.num_slots + kMaxDegree;
+    const int2* src = reinterpret_cast<const int2*>(c.qc_slot_packed);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tab[i] = src[i];
 }
 
 __device__ __forceinline__ int qc_addr(int jz, int r, int sh, int Z) {
@@ -203,8 +188,8 @@ __device__ __forceinline__ int qc_addr(int jz, int r, int sh, int Z) {
 // Whether every check of the lane's frame is satisfied; `test` says whether
 // this lane's frame is being checked.  Called by every lane (barriers).
 template <int P>
-__device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const float* a, const QcLane& l, bool test,
-                               int F, int* s_flag) {
+__device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const float* a,
+                               const QcLane& l, bool test, int F, int* s_flag) {
     const int Z = c.lifting;
     int bad = 0;
     if (test) {
@@ -214,14 +199,15 @@ __device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const float* a, const QcLa
             const int s0 = ld_uniform(c.qc_row_ptr, br);
             const int s1 = ld_uniform(c.qc_row_ptr, br + 1);
             unsigned parity = 0u;
-            for (int s = s0; s < s1; s += 4) {
-                float v[4];
-                int jz[4], sh[4];
-                slot_group<1, 4>(c, s, 0, jz, sh);
+            for (int s = s0; s < s1; s += 8) {
+                float v[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = a[qc_addr(jz[u], l.r, sh[u], Z)];
+                for (int u = 0; u < 8; ++u) {
+                    const int2 e = tab[s + u];  // padded: s + u < S + 32
+                    v[u] = a[qc_addr(e.x, l.r, e.y, Z)];
+                }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) parity ^= (s + u < s1 && v[u] < 0.0f) ? 1u : 0u;
+                for (int u = 0; u < 8; ++u) parity ^= (s + u < s1 && v[u] < 0.0f) ? 1u : 0u;
             }
             bad |= (br0 + l.p < c.base_rows) ? static_cast<int>(parity) : 0;
         }
@@ -250,17 +236,31 @@ __device__ __forceinline__ CheckState zero_state() {
 // phase, which leaves the APP alone.  Q and the LDS address of every edge of
 // the lane stay in registers between the two passes (MAXT = 32 / P slots).
 template <int P, bool UPDATE_APP>
-__device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, float* a,
+__device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int2* tab, float* a,
                                                 const CheckState st, int br, int r, int p,
                                                 const Params& prm) {
     constexpr int MAXT = kMaxDegree / P;
-    constexpr int G = 4;  // edges per group: a group's loads are issued together
+    constexpr int G = 8;  // slots per group; groups past the row's end are skipped
     const int Z = c.lifting;
     const int s0 = ld_uniform(c.qc_row_ptr, br);
     const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
     const int tmax = (dc + P - 1) / P;
     float q[MAXT];
     int addr[MAXT];
+    // Every load of the row first, LDS only (slot table, then APP), so they
+    // are in flight together.  Slots past the row's end inside the last group
+    // read real or padding table entries: their addresses stay in the frame.
+#pragma unroll
+    for (int g = 0; g < MAXT; g += G) {
+        if (g < tmax) {
+#pragma unroll
+            for (int t = g; t < g + G && t < MAXT; ++t) {
+                const int2 v = tab[s0 + t * P + p];
+                addr[t] = qc_addr(v.x, r, v.y, Z);
+                q[t] = a[addr[t]];
+            }
+        }
+    }
     float min1 = ck::kInf;
     float min2 = ck::kInf;
     int idx = 0x7fffffff;
@@ -268,17 +268,8 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, float* a,
 #pragma unroll
     for (int g = 0; g < MAXT; g += G) {
         if (g < tmax) {
-            // Slots past the row's end are clamped to real table entries, so
-            // their (unused) addresses stay inside the frame's APP.
-            int jz[G], sh[G];
-            slot_group<P, G>(c, s0 + g * P, p, jz, sh);
 #pragma unroll
-            for (int t = g; t < g + G; ++t) {
-                addr[t] = qc_addr(jz[t - g], r, sh[t - g], Z);
-                q[t] = a[addr[t]];
-            }
-#pragma unroll
-            for (int t = g; t < g + G; ++t) {
+            for (int t = g; t < g + G && t < MAXT; ++t) {
                 const int e = t * P + p;
                 const bool valid = e < dc;
                 const float qv = q[t] - state_msg(st, e);
@@ -314,7 +305,7 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, float* a,
     for (int g = 0; g < MAXT; g += G) {
         if (g < tmax) {
 #pragma unroll
-            for (int t = g; t < g + G; ++t) {
+            for (int t = g; t < g + G && t < MAXT; ++t) {
                 const int e = t * P + p;
                 if (e < dc) {
                     const unsigned ng = neg ^ ((q[t] < 0.0f) ? 1u : 0u);
@@ -342,7 +333,7 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, float* a,
 // messages of the row are loaded together up front; Q, phi(|Q|) and the LDS
 // address of every edge stay in registers between the passes.
 template <bool UPDATE_APP>
-__device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, float* a,
+__device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab, float* a,
                                           float* __restrict__ Rm, int br, int r, bool first) {
     constexpr int MAXT = kMaxDegree;
     const int Z = c.lifting;
@@ -360,9 +351,8 @@ __device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, float* a,
 #pragma unroll
     for (int t = 0; t < MAXT; ++t) {
         if (t < dc) {
-            int jz[1], sh[1];
-            slot_group<1, 1>(c, s0 + t, 0, jz, sh);
-            addr[t] = qc_addr(jz[0], r, sh[0], Z);
+            const int2 v = tab[s0 + t];
+            addr[t] = qc_addr(v.x, r, v.y, Z);
             q[t] = a[addr[t]] - rold[t];
             neg ^= (q[t] < 0.0f) ? 1u : 0u;
             ph[t] = phi(fabsf(q[t]));
@@ -387,11 +377,13 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm) {
     extern __shared__ float lds[];
     __shared__ int s_flag[kMaxFrames];
+    __shared__ int2 s_tab[kTabEntries];
     const int N = c.num_variables;
     const int Z = c.lifting;
     const int mb = c.base_rows;
     const QcLane l = qc_lane<P>(Z, F, R);
     float* a = lds + static_cast<size_t>(l.f < F ? l.f : 0) * N;
+    stage_slot_table(c, s_tab);
     load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
     __syncthreads();
 
@@ -411,7 +403,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
     bool ok = false;
     int iters = 0;
     if (prm.early_stop) {
-        ok = qc_syndrome_ok<P>(c, a, l, l.live, F, s_flag);
+        ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
         done = done || ok;
     }
     int it = 0;
@@ -423,7 +415,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
         for (int br = 0; br < mb; ++br) {
             if (!done) {
                 if (SP) {
-                    qc_sp_row<true>(c, a, Rm, br, l.r, first);
+                    qc_sp_row<true>(c, s_tab, a, Rm, br, l.r, first);
                 } else {
                     const CheckState st = st_next;
                     if (br + 1 < mb) {
@@ -432,7 +424,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
                     } else if (mb > 1) {
                         st_next = st_base[l.r];  // block row 0, stored this iteration
                     }
-                    const CheckState ns = qc_ms_row<P, true>(c, a, st, br, l.r, l.p, prm);
+                    const CheckState ns = qc_ms_row<P, true>(c, s_tab, a, st, br, l.r, l.p, prm);
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
                 }
@@ -442,14 +434,14 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
         ++it;
         if (!done) iters = it;
         if (prm.early_stop) {
-            const bool now = qc_syndrome_ok<P>(c, a, l, !done, F, s_flag);
+            const bool now = qc_syndrome_ok<P>(c, s_tab, a, l, !done, F, s_flag);
             if (!done) {
                 ok = now;
                 done = now;
             }
         }
     }
-    if (!prm.early_stop) ok = qc_syndrome_ok<P>(c, a, l, l.live, F, s_flag);
+    if (!prm.early_stop) ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
     __syncthreads();
     if (l.live)
         store_frame(c, a, l.frame, l.u, Z * P, iters, ok, app_out, bits_out, iters_out, valid_out);
@@ -462,10 +454,12 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm) {
     extern __shared__ float lds[];
     __shared__ int s_flag[kMaxFrames];
+    __shared__ int2 s_tab[kTabEntries];
     const int N = c.num_variables;
     const int Z = c.lifting;
     const QcLane l = qc_lane<P>(Z, F, R);
     float* a = lds + static_cast<size_t>(l.f < F ? l.f : 0) * N;
+    stage_slot_table(c, s_tab);
     load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
     __syncthreads();
 
@@ -478,7 +472,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     bool ok = false;
     int iters = 0;
     if (prm.early_stop) {
-        ok = qc_syndrome_ok<P>(c, a, l, l.live, F, s_flag);
+        ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
         done = done || ok;
     }
     int it = 0;
@@ -488,11 +482,11 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         if (!done) {
             for (int br = 0; br < c.base_rows; ++br) {
                 if (SP) {
-                    if (l.p == 0) qc_sp_row<false>(c, a, Rm, br, l.r, first);
+                    if (l.p == 0) qc_sp_row<false>(c, s_tab, a, Rm, br, l.r, first);
                 } else {
                     CheckState* sp = st_base + static_cast<size_t>(br) * Z + l.r;
                     const CheckState st = first ? zero_state() : *sp;
-                    const CheckState ns = qc_ms_row<P, false>(c, a, st, br, l.r, l.p, prm);
+                    const CheckState ns = qc_ms_row<P, false>(c, s_tab, a, st, br, l.r, l.p, prm);
                     if (l.p == 0) *sp = ns;
                 }
             }
@@ -525,14 +519,14 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         ++it;
         if (!done) iters = it;
         if (prm.early_stop) {
-            const bool now = qc_syndrome_ok<P>(c, a, l, !done, F, s_flag);
+            const bool now = qc_syndrome_ok<P>(c, s_tab, a, l, !done, F, s_flag);
             if (!done) {
                 ok = now;
                 done = now;
             }
         }
     }
-    if (!prm.early_stop) ok = qc_syndrome_ok<P>(c, a, l, l.live, F, s_flag);
+    if (!prm.early_stop) ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
     __syncthreads();
     if (l.live)
         store_frame(c, a, l.frame, l.u, Z * P, iters, ok, app_out, bits_out, iters_out, valid_out);
@@ -719,7 +713,8 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
     if (int32_t st = ck::device_info(&info)) return st;
     // Dynamic LDS budget: the per-workgroup limit less room for the kernels'
     // static LDS (frame flags, barrier scratch).
-    const size_t lds_max = info.lds_per_block > 2048 ? info.lds_per_block - 1024 : 0;
+    const size_t lds_max =
+        info.lds_per_block > kStaticLds + 2048 ? info.lds_per_block - kStaticLds - 1024 : 0;
     Plan p{};
     p.sp = m->rule == 1;
     p.flood = m->schedule == 1;
@@ -727,12 +722,12 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
     const int Z = c->lifting;
     const size_t app_bytes = static_cast<size_t>(N) * 4;
     bool qc_ok = Z > 0 && Z <= kMaxQcThreads && app_bytes <= lds_max &&
-                 (p.sp || c->max_row_degree <= kMaxDegree);
+                 c->max_row_degree <= kMaxDegree && c->num_slots <= kMaxSlots;
     if (m->kernel == 2 && !qc_ok)
         return ck::fail(CK_EINVAL,
                         "quasi-cyclic kernel requested but the code has no usable QC view "
-                        "(lifting %d, max row degree %d, APP %zu bytes, LDS %zu bytes)",
-                        Z, c->max_row_degree, app_bytes, lds_max);
+                        "(lifting %d, max row degree %d, %d slots, APP %zu bytes, LDS %zu bytes)",
+                        Z, c->max_row_degree, c->num_slots, app_bytes, lds_max);
     if (m->kernel == 1) qc_ok = false;
     if (qc_ok) {
         p.kind = 2;

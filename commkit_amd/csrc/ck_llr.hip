@@ -474,14 +474,14 @@ int32_t check_axis(const ck_axis* a, const char* name, bool closed) {
 
 template <bool CLOSED>
 int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_axis* im,
-                         float inv_s2, int exact, float* out, void* stream) {
+                         double inv_s2, int exact, float* out, void* stream) {
     CK_REQUIRE(n >= 0, "n = %lld < 0", static_cast<long long>(n));
     if (int32_t st = check_axis(re, "re", CLOSED)) return st;
     if (int32_t st = check_axis(im, "im", CLOSED)) return st;
     CK_REQUIRE(re->num_bits == im->num_bits,
                "separable axes must carry the same number of bits (%d, %d)", re->num_bits,
                im->num_bits);
-    CK_REQUIRE(inv_s2 > 0.0f && inv_s2 < ck::kInf, "inv_s2 must be positive and finite");
+    CK_REQUIRE(inv_s2 > 0.0 && inv_s2 < 1e300, "inv_s2 must be positive and finite");
     if (n == 0) return CK_OK;
     CK_REQUIRE(x != nullptr && out != nullptr, "NULL symbols or output");
     const int grid = grid_for(n);
@@ -491,8 +491,8 @@ int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_a
 #define CK_CASE(KB)                                                                           \
     case KB:                                                                                  \
         hipLaunchKernelGGL((llr_separable_kernel<KB, CLOSED>), dim3(grid), dim3(kThreads), 0, \
-                           s, xs, static_cast<long long>(n), *re, *im, inv_s2,                \
-                           static_cast<double>(inv_s2), exact, out);                          \
+                           s, xs, static_cast<long long>(n), *re, *im,                        \
+                           static_cast<float>(inv_s2), inv_s2, exact, out);                   \
         break;
         CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
 #undef CK_CASE
@@ -503,13 +503,13 @@ int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_a
 }  // namespace
 
 extern "C" int32_t ck_llr_separable_maxlog(const void* x, int64_t n, const ck_axis* re,
-                                           const ck_axis* im, float inv_s2, float* out,
+                                           const ck_axis* im, double inv_s2, float* out,
                                            void* stream) {
     return launch_separable<true>(x, n, re, im, inv_s2, 0, out, stream);
 }
 
 extern "C" int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
-                                    const ck_axis* im, float inv_s2, int32_t exact, float* out,
+                                    const ck_axis* im, double inv_s2, int32_t exact, float* out,
                                     void* stream) {
     CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
     return launch_separable<false>(x, n, re, im, inv_s2, exact, out, stream);
@@ -517,13 +517,13 @@ extern "C" int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
 
 extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,
                                  const double* log_prior, const int32_t* labels, int32_t M,
-                                 int32_t k, float inv_s2, int32_t exact, float* out,
+                                 int32_t k, double inv_s2, int32_t exact, float* out,
                                  void* stream) {
     CK_REQUIRE(n >= 0, "n = %lld < 0", static_cast<long long>(n));
     CK_REQUIRE(k >= 1 && k <= 16, "k = %d not in [1, 16]", k);
     CK_REQUIRE(M == (1 << k), "M = %d != 2^k", M);
     CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
-    CK_REQUIRE(inv_s2 > 0.0f && inv_s2 < ck::kInf, "inv_s2 must be positive and finite");
+    CK_REQUIRE(inv_s2 > 0.0 && inv_s2 < 1e300, "inv_s2 must be positive and finite");
     if (n == 0) return CK_OK;
     CK_REQUIRE(x && out && points && log_prior && labels, "NULL pointer argument");
     const int grid = grid_for(n);
@@ -535,8 +535,8 @@ extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,
 #define CK_CASE(K)                                                                           \
     case K:                                                                                  \
         hipLaunchKernelGGL((llr_points_kernel<K>), dim3(grid), dim3(kThreads), dyn, s, xs,   \
-                           static_cast<long long>(n), pts, log_prior, labels, M, inv_s2,     \
-                           static_cast<double>(inv_s2), exact, out);                         \
+                           static_cast<long long>(n), pts, log_prior, labels, M,             \
+                           static_cast<float>(inv_s2), inv_s2, exact, out);                  \
         break;
         CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
         CK_CASE(9) CK_CASE(10) CK_CASE(11) CK_CASE(12) CK_CASE(13) CK_CASE(14) CK_CASE(15)

@@ -168,7 +168,7 @@ def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
         )
     real_input = not x.is_complex()
     nv = 2.0 * noise_var if real_input else noise_var
-    inv_s2 = float(np.float32(1.0 / max(nv, 1e-20)))
+    inv_s2 = 1.0 / max(nv, 1e-20)
     shape = x.shape
     flat = x.reshape(-1)
     flat = flat.to(torch.complex64) if flat.dtype != torch.complex64 else flat.contiguous()
@@ -183,12 +183,12 @@ def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
                 if closed_form(c._axes, method):
                     _native.check(lib.ck_llr_separable_maxlog(
                         flat.data_ptr(), n, ctypes.byref(re_ax), ctypes.byref(im_ax),
-                        ctypes.c_float(inv_s2), out.data_ptr(), stream,
+                        ctypes.c_double(inv_s2), out.data_ptr(), stream,
                     ))  # fmt: skip
                 else:
                     _native.check(lib.ck_llr_separable(
                         flat.data_ptr(), n, ctypes.byref(re_ax), ctypes.byref(im_ax),
-                        ctypes.c_float(inv_s2), int(method == "exact"), out.data_ptr(),
+                        ctypes.c_double(inv_s2), int(method == "exact"), out.data_ptr(),
                         stream,
                     ))  # fmt: skip
             else:
@@ -202,7 +202,7 @@ def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
                 )  # fmt: skip
                 _native.check(lib.ck_llr_points(
                     flat.data_ptr(), n, pts.data_ptr(), lp.data_ptr(), lab.data_ptr(),
-                    c.order, k, ctypes.c_float(inv_s2), int(method == "exact"),
+                    c.order, k, ctypes.c_double(inv_s2), int(method == "exact"),
                     out.data_ptr(), stream,
                 ))  # fmt: skip
     return out.reshape(*shape[:-1], shape[-1] * k) if len(shape) else out.reshape(-1)

@@ -64,12 +64,14 @@ typedef struct ck_axis {
 
 /* Closed-form max-log LLRs of a uniform separable lattice.
  * x: (n,) complex64 device; out: (n, k) float32 device, k = re.num_bits +
- * im.num_bits.  inv_s2 = 1 / sigma^2.
+ * im.num_bits.  inv_s2 = 1 / sigma^2 (float64: with a prior, the likelihood
+ * and prior terms of a small LLR cancel, and a float32 1/sigma^2 would
+ * cost its rounding relative to the larger term).
  * Replaces the kernel `llr_separable_maxlog` (commkit/_cuda/src/llr.cu:
  * 304-337) reached through _cuda.get_kernel("llr_separable_maxlog")
  * (commkit/_cuda/llr.py:150; call site commkit/mapping/_demap.py:425). */
 int32_t ck_llr_separable_maxlog(const void* x, int64_t n, const ck_axis* re,
-                                const ck_axis* im, float inv_s2, float* out,
+                                const ck_axis* im, double inv_s2, float* out,
                                 void* stream);
 
 /* LLRs from the metrics of each axis's levels, with the log prior:
@@ -77,7 +79,7 @@ int32_t ck_llr_separable_maxlog(const void* x, int64_t n, const ck_axis* re,
  * Replaces `llr_separable` (commkit/_cuda/src/llr.cu:225-259; wrapper
  * commkit/_cuda/llr.py:87; call site commkit/mapping/_demap.py:444). */
 int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
-                         const ck_axis* im, float inv_s2, int32_t exact,
+                         const ck_axis* im, double inv_s2, int32_t exact,
                          float* out, void* stream);
 
 /* LLRs from the metrics of all M points (any constellation).
@@ -88,7 +90,7 @@ int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
  * commkit/_cuda/llr.py; call site commkit/mapping/_demap.py:305). */
 int32_t ck_llr_points(const void* x, int64_t n, const void* points,
                       const double* log_prior, const int32_t* labels, int32_t M,
-                      int32_t k, float inv_s2, int32_t exact, float* out,
+                      int32_t k, double inv_s2, int32_t exact, float* out,
                       void* stream);
 
 /* ------------------------------------------------------------------------

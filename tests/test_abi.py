@@ -52,13 +52,13 @@ def test_struct_layouts_match_header():
 
 def test_argument_errors_return_status_and_message():
     lib = _native.lib()
-    st = lib.ck_llr_separable_maxlog(None, 10, None, None, ctypes.c_float(1.0), None, None)
+    st = lib.ck_llr_separable_maxlog(None, 10, None, None, ctypes.c_double(1.0), None, None)
     assert st == 1 and b"axis is NULL" in lib.ck_last_error()
     ax = CkAxis(1, 1, 1, 1, 1, 1, 8, 3)
-    st = lib.ck_llr_separable(None, 4, ctypes.byref(ax), ctypes.byref(ax), ctypes.c_float(1.0), 2,
+    st = lib.ck_llr_separable(None, 4, ctypes.byref(ax), ctypes.byref(ax), ctypes.c_double(1.0), 2,
                               None, None)  # fmt: skip
     assert st == 1 and b"exact must be 0 or 1" in lib.ck_last_error()
-    st = lib.ck_llr_points(None, 4, None, None, None, 16, 3, ctypes.c_float(1.0), 0, None, None)
+    st = lib.ck_llr_points(None, 4, None, None, None, 16, 3, ctypes.c_double(1.0), 0, None, None)
     assert st == 1 and b"M = 16 != 2^k" in lib.ck_last_error()
     st = lib.ck_ldpc_decode(ctypes.byref(CkLdpcCode()), None, None, None, None, None, None, 1,
                             None, None)  # fmt: skip

@@ -5,8 +5,8 @@ definition (float64 constellation, float64 arithmetic, the same complex64
 symbols), a GPU LLR L32 meets
 
 - max-log: ``|L32 - L64| <= 1e-5 * |L64|`` on every element, with no floor
-  (the kernels form metric differences in float64 from float64 tables, so
-  only the float32 ``1/sigma^2`` and the float32 output round);
+  (the kernels form metric differences in float64 from float64 tables and
+  a float64 ``1/sigma^2``, so only the float32 output rounds);
 - exact (log-sum-exp): ``|L32 - L64| <= 1e-5 * max(|L64|, 2^-5)``: relative
   1e-5 except within 2^-5 of zero, where the float32 correction sums
   ``log(1 + T)`` allow an absolute 3e-7.
