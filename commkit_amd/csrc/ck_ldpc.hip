@@ -261,10 +261,15 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
             }
         }
     }
+    // Pass 1.  min1 / min2 / idx follow the sequential rule exactly:
+    //   idx moves on a strictly smaller |Q| (the first edge attaining min1),
+    //   min2 = min(min2, max(min1, |Q|)), min1 = min(min1, |Q|)
+    // (min and max return one of their operands, so no value is rounded).
+    // Bit e of qneg is Q_e < 0; the sign parity is its popcount.
     float min1 = ck::kInf;
     float min2 = ck::kInf;
     int idx = 0x7fffffff;
-    unsigned neg = 0u;
+    unsigned qneg = 0u;
 #pragma unroll
     for (int g = 0; g < MAXT; g += G) {
         if (g < tmax) {
@@ -275,12 +280,10 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
                 const float qv = q[t] - state_msg(st, e);
                 q[t] = qv;
                 const float mag = valid ? fabsf(qv) : ck::kInf;
-                const bool lt1 = mag < min1;
-                const bool lt2 = mag < min2;
-                min2 = lt1 ? min1 : (lt2 ? mag : min2);
-                idx = lt1 ? e : idx;
-                min1 = lt1 ? mag : min1;
-                neg ^= (valid && qv < 0.0f) ? 1u : 0u;
+                idx = (mag < min1) ? e : idx;
+                min2 = fminf(min2, fmaxf(min1, mag));
+                min1 = fminf(min1, mag);
+                qneg |= (valid && qv < 0.0f) ? (1u << e) : 0u;
             }
         }
     }
@@ -289,7 +292,7 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
         const float o1 = __shfl_xor(min1, off);
         const float o2 = __shfl_xor(min2, off);
         const int oi = __shfl_xor(idx, off);
-        neg ^= static_cast<unsigned>(__shfl_xor(static_cast<int>(neg), off));
+        qneg |= static_cast<unsigned>(__shfl_xor(static_cast<int>(qneg), off));
         if (o1 < min1 || (o1 == min1 && oi < idx)) {
             min2 = fminf(min1, o2);
             min1 = o1;
@@ -300,27 +303,24 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
     }
     const float m1 = fmaxf(prm.scale * min1 - prm.offset, 0.0f);
     const float m2 = fmaxf(prm.scale * min2 - prm.offset, 0.0f);
-    unsigned signs = 0u;
+    // Pass 2: R_e is negative where (total parity) ^ (Q_e < 0).
+    const unsigned row_mask = (dc >= 32) ? 0xffffffffu : ((1u << dc) - 1u);
+    const unsigned signs = ((__popc(qneg) & 1) ? ~qneg : qneg) & row_mask;
+    if (UPDATE_APP) {
 #pragma unroll
-    for (int g = 0; g < MAXT; g += G) {
-        if (g < tmax) {
+        for (int g = 0; g < MAXT; g += G) {
+            if (g < tmax) {
 #pragma unroll
-            for (int t = g; t < g + G && t < MAXT; ++t) {
-                const int e = t * P + p;
-                if (e < dc) {
-                    const unsigned ng = neg ^ ((q[t] < 0.0f) ? 1u : 0u);
-                    if (UPDATE_APP) {
+                for (int t = g; t < g + G && t < MAXT; ++t) {
+                    const int e = t * P + p;
+                    if (e < dc) {
                         const float mag = (e == idx) ? m2 : m1;
-                        a[addr[t]] = q[t] + (ng ? -mag : mag);
+                        a[addr[t]] = q[t] + (((signs >> e) & 1u) ? -mag : mag);
                     }
-                    signs |= ng << e;
                 }
             }
         }
     }
-#pragma unroll
-    for (int off = 1; off < P; off <<= 1)
-        signs |= static_cast<unsigned>(__shfl_xor(static_cast<int>(signs), off));
     CheckState out;
     out.m1 = m1;
     out.m2 = m2;
