@@ -36,6 +36,12 @@ SYMBOLS: dict[str, tuple[Any, list[Any]]] = {
     "ck_llr_points": (_i32, [_p, _i64, _p, _p, _p, _i32, _i32, _f64, _i32, _p, _p]),
     "ck_ldpc_workspace_bytes": (_i64, [_p, _p, _i64]),
     "ck_ldpc_decode": (_i32, [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _p]),
+    "ck_batch_workspace_bytes": (_i64, [_p, _i32, _p]),
+    "ck_batch_prepare": (_i32, [_p, _i32, _p, _p, _i64, _p]),
+    "ck_batch_launch": (_i32, [_p, _p]),
+    "ck_batch_graph_create": (_i32, [_p, _p]),
+    "ck_batch_graph_launch": (_i32, [_p, _p]),
+    "ck_batch_graph_destroy": (_i32, [_p]),
 }
 
 _LIB: ctypes.CDLL | None = None

@@ -38,6 +38,7 @@ COMMON = [
 
 PER_FILE = {
     "ck_ldpc.hip": ["-ffp-contract=off"],
+    "ck_batch.hip": ["-ffp-contract=off"],
 }
 
 

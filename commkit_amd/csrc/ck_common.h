@@ -37,6 +37,10 @@ int32_t device_info(DeviceInfo* out);
 
 constexpr float kInf = __builtin_huge_valf();
 
+// Validation of an LDPC code / method (ck_ldpc.hip); CK_OK or CK_EINVAL.
+int32_t check_code(const ck_ldpc_code* c);
+int32_t check_method(const ck_ldpc_method* m);
+
 }  // namespace ck
 
 #define CK_REQUIRE(cond, ...)                          \

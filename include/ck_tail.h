@@ -172,6 +172,61 @@ int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method* method,
                        int32_t* iterations, uint8_t* valid, int64_t R,
                        void* workspace, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched decoding: frames of several QC codes in one persistent launch
+ * ------------------------------------------------------------------------ */
+
+/* The frames of one code: inputs and outputs as for ck_ldpc_decode, frames
+ * contiguous.  frames_per_group / lanes_per_row: 0 lets the library choose. */
+typedef struct ck_batch_class {
+    const ck_ldpc_code* code;  /* host struct; its tables stay valid while
+                                  the batch is used                           */
+    int64_t num_frames;
+    const float* llrs;         /* (num_frames, n) device                      */
+    float* app;                /* (num_frames, n) device                      */
+    int8_t* bits;              /* (num_frames, k) device, or NULL             */
+    int32_t* iterations;       /* (num_frames) device                         */
+    uint8_t* valid;            /* (num_frames) device                         */
+    int32_t frames_per_group;  /* frame slots per workgroup                   */
+    int32_t lanes_per_row;     /* 1 or 2                                      */
+} ck_batch_class;
+
+/* Launch plan filled by ck_batch_prepare. */
+typedef struct ck_batch_plan {
+    void* workspace;
+    int32_t grid;
+    int32_t threads;
+    int32_t lds_bytes;
+    int32_t sum_product;
+    int32_t frames_per_group[8];
+} ck_batch_plan;
+
+/* Device workspace for a batch of up to 8 classes (-1 on bad arguments).
+ * No counterpart in the reference, which decodes one code per launch in
+ * chunks (commkit/coding/ldpc.py:565-581); the batch generalises that loop
+ * to uneven codes in one launch. */
+int64_t ck_batch_workspace_bytes(const ck_batch_class* classes,
+                                 int32_t num_classes,
+                                 const ck_ldpc_method* method);
+
+/* Writes the batch's descriptor into the workspace (a synchronous copy) and
+ * fills the launch plan.  Layered min-sum or sum-product.  Classes should be
+ * ordered by cost, largest frames first. */
+int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_classes,
+                         const ck_ldpc_method* method, void* workspace,
+                         int64_t workspace_bytes, ck_batch_plan* plan);
+
+/* Decodes every frame of the prepared batch: a counter-reset kernel, then
+ * the persistent decoder.  Reads and writes only the buffers named at
+ * prepare time, so it may be captured in a HIP graph. */
+int32_t ck_batch_launch(const ck_batch_plan* plan, void* stream);
+
+/* A HIP graph of ck_batch_launch, instantiated; launch it any number of
+ * times, then destroy it. */
+int32_t ck_batch_graph_create(const ck_batch_plan* plan, void** graph);
+int32_t ck_batch_graph_launch(void* graph, void* stream);
+int32_t ck_batch_graph_destroy(void* graph);
+
 #ifdef __cplusplus
 }
 #endif
