@@ -34,6 +34,13 @@ ORDER = 64
 FRAMES = 4096
 MARGIN_DB = 0.3
 
+# HBM bytes per decoder launch on the default workload, from FETCH_SIZE and
+# WRITE_SIZE counters scaled by a plain copy of known size
+# (profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md).  Counters cannot be
+# read inside this process, so the figure is reported only for the workload
+# it was measured on.
+PROFILED_TRAFFIC = {"frames": FRAMES, "bytes": 2.218e9}
+
 
 def _parse() -> argparse.Namespace:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
@@ -45,8 +52,40 @@ def _parse() -> argparse.Namespace:
                     help="override Es/N0 (dB); default: measured waterfall + 0.3 dB")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="target CPU time of the baseline sample")
+                    help="target CPU time of the baseline sample (5 timed runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device-to-device twice (counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_bandwidth(dev, nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Measured device-to-device copy rate (read + write bytes per second, GB/s)."""
+    import torch
+
+    x = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    b.synchronize()
+    return 2.0 * nbytes * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+
+
+def copy_calibration(dev) -> None:
+    """A plain copy of known size (256 MiB read, 256 MiB written) for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of the decoder's profile."""
+    import torch
+
+    x = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {x.numel()} bytes twice", flush=True)
 
 
 def decoder_bytes(code, iters: np.ndarray) -> float:
@@ -72,7 +111,6 @@ def cpu_baseline(code, constellation, rx_host: np.ndarray, info_host: np.ndarray
         closed.append((a.levels, pack_bits(a.labels), lo, hi, a.bits))
     inv = float(np.float32(1.0 / noise_var))
     row_ptr, cols, _ = code._schedule
-    sym_per_frame = code.n // constellation.bits_per_symbol
 
     def run(frames: int) -> float:
         t0 = time.perf_counter()
@@ -85,19 +123,27 @@ def cpu_baseline(code, constellation, rx_host: np.ndarray, info_host: np.ndarray
         return time.perf_counter() - t0
 
     probe = min(16, rx_host.shape[0])
+    run(probe)  # warm-up
     t = run(probe)
-    frames = int(min(rx_host.shape[0], max(probe, probe * target_s / max(t, 1e-3))))
-    times = [run(frames) for _ in range(1)]
+    per_run = target_s / 5
+    frames = int(min(rx_host.shape[0], max(probe, probe * per_run / max(t, 1e-3))))
+    times = [run(frames) for _ in range(5)]
     dt = float(np.median(times))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    del sym_per_frame
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {
         "value": frames * code.k / dt / 1e9,
         "unit": "Gb/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{frames} of the {rx_host.shape[0]} frames (max-log LLRs + layered "
-                  f"min-sum, C/OpenMP oracle, {threads} threads), {dt:.2f} s",
+                  f"min-sum, C/OpenMP oracle, {threads} threads of {os.cpu_count()} on "
+                  f"{model}), median of 5 runs, {dt:.2f} s each",
     }
 
 
@@ -118,6 +164,9 @@ def main() -> int:
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        copy_calibration(dev)
+        return 0
     code = seeded_code(CODE)
     c = Constellation.qam(ORDER)
     esn0 = args.esn0 if args.esn0 is not None else round(waterfall_db(CODE, ORDER) + MARGIN_DB, 3)
@@ -197,7 +246,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (PROFILED_TRAFFIC["bytes"] if F == PROFILED_TRAFFIC["frames"]
+                        and args.esn0 is None else None),
             "kernel": "ldpc_qc_layered (decode)",
             "bytes_per_launch": dec_bytes,
         },
@@ -207,6 +257,12 @@ def main() -> int:
             "unit": "GB/s",
             "bytes_per_launch": demap_bytes,
         },
+        "tail_roofline": {
+            "achieved": round((demap_bytes + dec_bytes) / ((demap_ms + decode_ms) * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+        },
+        "hbm_copy_GBs": round(copy_bandwidth(dev), 1),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(code, c, fr.rx.cpu().numpy(), fr.info.cpu().numpy(),
