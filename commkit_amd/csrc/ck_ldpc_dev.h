@@ -47,8 +47,8 @@ __host__ __device__ __forceinline__ size_t state_offset(int F, int N) {
 }
 
 __device__ __forceinline__ float state_msg(const CheckState& st, int e) {
-    const float mag = (e == st.idx) ? st.m2 : st.m1;
-    return ((st.signs >> e) & 1u) ? -mag : mag;
+    const float mag = (e == st.idx) ? st.m2 : st.m1;  // m1, m2 >= +0
+    return __uint_as_float(__float_as_uint(mag) | ((st.signs << (31 - e)) & 0x80000000u));
 }
 
 // Channel LLRs of frames [frame0, frame0 + F) into LDS (zeros for the
@@ -133,18 +133,35 @@ __device__ __forceinline__ T ld_uniform(const T* p, int i) {
     return ((const CK_CONSTANT T*)p)[i];
 }
 
-// Copies the packed {block column * Z, shift} slot table (S + 32 entries,
-// zero padded) into LDS, where every row update reads it.
+// Stages the slot table in LDS, where every row update reads it.  The code
+// holds {block column * Z, shift} per nonzero circulant (S + 32 entries, zero
+// padded); lane r of a block row reaches variable jZ + (r + shift) mod Z of
+// slot {jZ, shift}, which in bytes from &a[r] is
+//   4 (jZ + shift) - (r >= Z - shift ? 4Z : 0),
+// so the LDS copy holds {4 (jZ + shift), Z - shift}.  Padding {0, 0} becomes
+// {0, Z}: offset 0, the lane's own row, inside the frame.
 __device__ inline void stage_slot_table(const ck_ldpc_code& c, int2* tab) {
     const int n = c.num_slots + kMaxDegree;
+    const int Z = c.lifting;
     const int2* src = reinterpret_cast<const int2*>(c.qc_slot_packed);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tab[i] = src[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int2 v = src[i];
+        tab[i] = make_int2(4 * (v.x + v.y), Z - v.y);
+    }
 }
 
-__device__ __forceinline__ int qc_addr(int jz, int r, int sh, int Z) {
-    int col = r + sh;
-    col -= (col >= Z) ? Z : 0;
-    return jz + col;
+// Byte offset from &a[r] of lane r's variable in the staged slot `v`.
+__device__ __forceinline__ int qc_off(int2 v, int r, int z4) {
+    return v.x + ((r >= v.y) ? -z4 : 0);
+}
+
+__device__ __forceinline__ float& lds_at(float* ar, int off) {
+    return *reinterpret_cast<float*>(reinterpret_cast<char*>(ar) + off);
+}
+
+// `mag` (sign bit clear) with the sign of bit e of `bits`.
+__device__ __forceinline__ float with_sign(float mag, unsigned bits, int e) {
+    return __uint_as_float(__float_as_uint(mag) | ((bits << (31 - e)) & 0x80000000u));
 }
 
 // Whether every check of the lane's frame is satisfied; `test` says whether
@@ -152,7 +169,8 @@ __device__ __forceinline__ int qc_addr(int jz, int r, int sh, int Z) {
 template <int P>
 __device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const float* a,
                                const QcLane& l, bool test, int F, int* s_flag) {
-    const int Z = c.lifting;
+    const int z4 = 4 * c.lifting;
+    float* ar = const_cast<float*>(a) + l.r;
     int bad = 0;
     if (test) {
         // Lane p of a row checks the rows of block rows p, p+P, ...
@@ -166,7 +184,7 @@ __device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const flo
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const int2 e = tab[s + u];  // padded: s + u < S + 32
-                    v[u] = a[qc_addr(e.x, l.r, e.y, Z)];
+                    v[u] = lds_at(ar, qc_off(e, l.r, z4));
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) parity ^= (s + u < s1 && v[u] < 0.0f) ? 1u : 0u;
@@ -193,61 +211,47 @@ __device__ __forceinline__ CheckState zero_state() {
     return z;
 }
 
-// One min-sum check row of a block row, given its old state; returns the new
-// state.  UPDATE_APP: layered (APP = Q + R_new); else the flooding check
-// phase, which leaves the APP alone.  Q and the LDS address of every edge of
-// the lane stay in registers between the two passes (MAXT = 32 / P slots).
-template <int P, bool UPDATE_APP>
-__device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int2* tab, float* a,
-                                                const CheckState st, int br, int r, int p,
-                                                const Params& prm) {
-    constexpr int MAXT = kMaxDegree / P;
-    constexpr int G = 8;  // slots per group; groups past the row's end are skipped
-    const int Z = c.lifting;
-    const int s0 = ld_uniform(c.qc_row_ptr, br);
-    const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
-    const int tmax = (dc + P - 1) / P;
-    float q[MAXT];
-    int addr[MAXT];
-    // Every load of the row first, LDS only (slot table, then APP), so they
-    // are in flight together.  Slots past the row's end inside the last group
-    // read real or padding table entries: their addresses stay in the frame.
+// One min-sum check row of a block row over NT slots of the lane (edges
+// e = t*P + p), given its old state; returns the new state.  Branch-free:
+// every slot is processed, and only the last four can lie past the row's end
+// (their table entries are real or zero padding, so their addresses stay in
+// the frame).  UPDATE_APP: layered (APP = Q + R_new); else the flooding check
+// phase, which leaves the APP alone.  Q and the LDS address of every edge
+// stay in registers between the two passes.
+template <int P, int NT, bool UPDATE_APP>
+__device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, const CheckState st,
+                                                  int s0, int dc, int r, int p, int Z,
+                                                  const Params& prm) {
+    float q[NT];
+    float* addr[NT];
+    float* ar = a + r;
+    const int z4 = 4 * Z;
 #pragma unroll
-    for (int g = 0; g < MAXT; g += G) {
-        if (g < tmax) {
-#pragma unroll
-            for (int t = g; t < g + G && t < MAXT; ++t) {
-                const int2 v = tab[s0 + t * P + p];
-                addr[t] = qc_addr(v.x, r, v.y, Z);
-                q[t] = a[addr[t]];
-            }
-        }
+    for (int t = 0; t < NT; ++t) {
+        addr[t] = &lds_at(ar, qc_off(tab[s0 + t * P + p], r, z4));
+        q[t] = *addr[t];
     }
     // Pass 1.  min1 / min2 / idx follow the sequential rule exactly:
     //   idx moves on a strictly smaller |Q| (the first edge attaining min1),
-    //   min2 = min(min2, max(min1, |Q|)), min1 = min(min1, |Q|)
-    // (min and max return one of their operands, so no value is rounded).
+    //   min2 = min(min2, max(min1, |Q|)) = med3(min1, min2, |Q|) as min1 <= min2,
+    //   min1 = min(min1, |Q|)
+    // (min, max and med3 return one of their operands, so nothing is rounded).
     // Bit e of qneg is Q_e < 0; the sign parity is its popcount.
     float min1 = ck::kInf;
     float min2 = ck::kInf;
     int idx = 0x7fffffff;
     unsigned qneg = 0u;
 #pragma unroll
-    for (int g = 0; g < MAXT; g += G) {
-        if (g < tmax) {
-#pragma unroll
-            for (int t = g; t < g + G && t < MAXT; ++t) {
-                const int e = t * P + p;
-                const bool valid = e < dc;
-                const float qv = q[t] - state_msg(st, e);
-                q[t] = qv;
-                const float mag = valid ? fabsf(qv) : ck::kInf;
-                idx = (mag < min1) ? e : idx;
-                min2 = fminf(min2, fmaxf(min1, mag));
-                min1 = fminf(min1, mag);
-                qneg |= (valid && qv < 0.0f) ? (1u << e) : 0u;
-            }
-        }
+    for (int t = 0; t < NT; ++t) {
+        const int e = t * P + p;
+        const bool valid = (t < NT - 4) || (e < dc);
+        const float qv = q[t] - state_msg(st, e);
+        q[t] = qv;
+        const float mag = valid ? fabsf(qv) : ck::kInf;
+        idx = (mag < min1) ? e : idx;
+        min2 = __builtin_amdgcn_fmed3f(min1, min2, mag);
+        min1 = fminf(min1, mag);
+        qneg |= (valid && qv < 0.0f) ? (1u << e) : 0u;
     }
 #pragma unroll
     for (int off = 1; off < P; off <<= 1) {
@@ -270,16 +274,11 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
     const unsigned signs = ((__popc(qneg) & 1) ? ~qneg : qneg) & row_mask;
     if (UPDATE_APP) {
 #pragma unroll
-        for (int g = 0; g < MAXT; g += G) {
-            if (g < tmax) {
-#pragma unroll
-                for (int t = g; t < g + G && t < MAXT; ++t) {
-                    const int e = t * P + p;
-                    if (e < dc) {
-                        const float mag = (e == idx) ? m2 : m1;
-                        a[addr[t]] = q[t] + (((signs >> e) & 1u) ? -mag : mag);
-                    }
-                }
+        for (int t = 0; t < NT; ++t) {
+            const int e = t * P + p;
+            if ((t < NT - 4) || (e < dc)) {
+                const float mag = (e == idx) ? m2 : m1;
+                *addr[t] = q[t] + with_sign(mag, signs, e);
             }
         }
     }
@@ -289,6 +288,31 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
     out.signs = signs;
     out.idx = idx;
     return out;
+}
+
+// Chooses the instantiation of qc_ms_row_n by the row's slot count, rounded
+// up to a multiple of four (the count is uniform across the workgroup).
+template <int P, bool UPDATE_APP, int G = 1>
+__device__ __forceinline__ CheckState qc_ms_dispatch(int groups, const int2* tab, float* a,
+                                                     const CheckState st, int s0, int dc, int r,
+                                                     int p, int Z, const Params& prm) {
+    if constexpr (4 * G * P >= kMaxDegree) {
+        return qc_ms_row_n<P, 4 * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
+    } else {
+        if (groups <= G) return qc_ms_row_n<P, 4 * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
+        return qc_ms_dispatch<P, UPDATE_APP, G + 1>(groups, tab, a, st, s0, dc, r, p, Z, prm);
+    }
+}
+
+// One min-sum check row `r` of block row `br`.
+template <int P, bool UPDATE_APP>
+__device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int2* tab, float* a,
+                                                const CheckState st, int br, int r, int p,
+                                                const Params& prm) {
+    const int s0 = ld_uniform(c.qc_row_ptr, br);
+    const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
+    const int groups = ((dc + P - 1) / P + 3) >> 2;
+    return qc_ms_dispatch<P, UPDATE_APP>(groups, tab, a, st, s0, dc, r, p, c.lifting, prm);
 }
 
 // One sum-product check row; Rm is the frame's messages, [slot][Z].  The old
@@ -310,12 +334,13 @@ __device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab
     int addr[MAXT];
     float total = 0.0f;
     unsigned neg = 0u;
+    float* ar = a + r;
+    const int z4 = 4 * Z;
 #pragma unroll
     for (int t = 0; t < MAXT; ++t) {
         if (t < dc) {
-            const int2 v = tab[s0 + t];
-            addr[t] = qc_addr(v.x, r, v.y, Z);
-            q[t] = a[addr[t]] - rold[t];
+            addr[t] = qc_off(tab[s0 + t], r, z4);
+            q[t] = lds_at(ar, addr[t]) - rold[t];
             neg ^= (q[t] < 0.0f) ? 1u : 0u;
             ph[t] = phi(fabsf(q[t]));
             total = total + ph[t];
@@ -326,7 +351,7 @@ __device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab
         if (t < dc) {
             const float mag = phi(total - ph[t]);
             const float rn = (neg ^ ((q[t] < 0.0f) ? 1u : 0u)) ? -mag : mag;
-            if (UPDATE_APP) a[addr[t]] = q[t] + rn;
+            if (UPDATE_APP) lds_at(ar, addr[t]) = q[t] + rn;
             Rm[static_cast<size_t>(s0 + t) * Z + r] = rn;
         }
     }
