@@ -71,7 +71,8 @@ __global__ void batch_reset(BatchDesc* d) {
 // once the frames already satisfied (0 iterations), asking again.
 template <int P>
 __device__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_code& c,
-                       const int2* tab, float* a, const QcLane& l, bool lane_ok, int F,
+                       const int2* tab, float* lds, unsigned* bits, float* a, const QcLane& l,
+                       bool lane_ok, int F,
                        const Params& prm, int* s_need, int* s_frame, int* s_flag, bool& active,
                        int& my_it, long long& frame) {
     const int Z = c.lifting;
@@ -94,7 +95,7 @@ __device__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_co
         if (lane_ok && l.u == 0) s_need[l.f] = 0;
         bool done0 = false;
         if (prm.early_stop) {
-            const bool ok = qc_syndrome_ok<P>(c, tab, a, l, fresh, F, s_flag);
+            const bool ok = qc_syndrome_ok(c, tab, lds, bits, l, lane_ok, fresh, F, s_flag);
             done0 = fresh && ok;
             if (done0) store_frame(c, a, frame, l.u, Z * P, 0, true, bc.app, bc.bits, bc.iters, bc.valid);
         }
@@ -122,6 +123,7 @@ __device__ void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s
     const bool lane_ok = l.f < F;
     const int slot = lane_ok ? l.f : 0;
     float* a = lds + static_cast<size_t>(slot) * N;
+    unsigned* bits = reinterpret_cast<unsigned*>(lds + qc_lds(F, N, Z, c.num_checks, false).bits);
     char* ws = d->slots + static_cast<size_t>(blockIdx.x) * d->slot_bytes;
     CheckState* st_base = reinterpret_cast<CheckState*>(ws) + static_cast<size_t>(slot) * c.num_checks;
     float* Rm = reinterpret_cast<float*>(ws) + static_cast<size_t>(slot) * c.num_edges;
@@ -132,7 +134,7 @@ __device__ void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s
     bool active = false;
     int my_it = 0;
     long long frame = -1;
-    refill<P>(bc, counter, c, s_tab, a, l, lane_ok, F, prm, s_need, s_frame, s_flag, active, my_it,
+    refill<P>(bc, counter, c, s_tab, lds, bits, a, l, lane_ok, F, prm, s_need, s_frame, s_flag, active, my_it,
               frame);
     CheckState st_next = zero_state();
     while (__syncthreads_or(active)) {
@@ -157,14 +159,14 @@ __device__ void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s
         }
         if (active) ++my_it;
         const bool check = active && (prm.early_stop || my_it >= prm.max_iterations);
-        const bool ok = qc_syndrome_ok<P>(c, s_tab, a, l, check, F, s_flag);
+        const bool ok = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, check, F, s_flag);
         const bool fin = active && ((prm.early_stop && ok) || my_it >= prm.max_iterations);
         if (fin) {
             store_frame(c, a, frame, l.u, Z * P, my_it, ok, bc.app, bc.bits, bc.iters, bc.valid);
             active = false;
             if (l.u == 0) s_need[l.f] = 1;
         }
-        refill<P>(bc, counter, c, s_tab, a, l, lane_ok, F, prm, s_need, s_frame, s_flag, active,
+        refill<P>(bc, counter, c, s_tab, lds, bits, a, l, lane_ok, F, prm, s_need, s_frame, s_flag, active,
                   my_it, frame);
     }
 }
@@ -222,7 +224,11 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
                    static_cast<long long>(bc.num_frames));
         CK_REQUIRE(bc.num_frames == 0 || (bc.llrs && bc.app && bc.iterations && bc.valid),
                    "class %d: NULL input or output", i);
-        const size_t app_bytes = static_cast<size_t>(c.num_variables) * 4;
+        CK_REQUIRE(syn_words(c.lifting, c.num_variables) < (1u << 21),
+                   "class %d: %d block columns of lifting %d are too many", i,
+                   c.num_variables / c.lifting, c.lifting);
+        // APP plus sign bitmaps of one frame.
+        const size_t app_bytes = qc_lds(1, c.num_variables, c.lifting, c.num_checks, false).end * 4;
         int P = bc.lanes_per_row;
         if (P == 0) {
             // Two lanes per row when LDS, not lanes, limits the slots.
@@ -234,15 +240,18 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
         CK_REQUIRE(P == 1 || P == 2, "class %d: lanes_per_row %d not 1 or 2", i, P);
         CK_REQUIRE(c.lifting * P <= kBatchThreads, "class %d: lifting %d x %d lanes above %d", i,
                    c.lifting, P, kBatchThreads);
-        const size_t app = static_cast<size_t>(c.num_variables) * 4;
         int F = bc.frames_per_group;
         if (F == 0) F = kBatchThreads / (c.lifting * P);
         F = std::min({F, kBatchThreads / (c.lifting * P), kMaxFrames,
-                      static_cast<int>(lds_max / app)});
-        CK_REQUIRE(F >= 1, "class %d: a frame (%zu bytes of APP) does not fit in LDS", i, app);
+                      static_cast<int>(lds_max / app_bytes)});
+        CK_REQUIRE(F >= 1, "class %d: a frame (%zu bytes of APP and bitmaps) does not fit in LDS",
+                   i, app_bytes);
+        while (F > 1 && qc_lds(F, c.num_variables, c.lifting, c.num_checks, false).end * 4 > lds_max)
+            --F;
         L.F[i] = F;
         L.P[i] = P;
-        L.lds_bytes = std::max(L.lds_bytes, static_cast<size_t>(F) * app);
+        L.lds_bytes = std::max(L.lds_bytes,
+                               qc_lds(F, c.num_variables, c.lifting, c.num_checks, false).end * 4);
         const size_t per_slot = sp ? static_cast<size_t>(c.num_edges) * 4
                                    : static_cast<size_t>(c.num_checks) * sizeof(CheckState);
         L.slot_bytes = std::max(L.slot_bytes, static_cast<size_t>(F) * per_slot);

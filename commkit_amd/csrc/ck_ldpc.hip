@@ -50,8 +50,11 @@ namespace {
 
 using namespace ckdev;
 
-template <bool SP, int P, bool STATE_LDS>
-__global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
+// SHORT (min-sum, one lane per row, rows of at most kShortRow edges): at
+// most 128 VGPRs, so that two workgroups of up to six waves each fit a CU.
+template <bool SP, int P, bool STATE_LDS, bool SHORT>
+__global__ __launch_bounds__(kMaxQcThreads)
+__attribute__((amdgpu_waves_per_eu(SHORT ? 4 : 1))) void ldpc_qc_layered(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm) {
@@ -67,12 +70,15 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
     load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
     __syncthreads();
 
+    const QcLds lay = qc_lds(F, N, Z, c.num_checks, STATE_LDS);
+    unsigned* bits = reinterpret_cast<unsigned*>(lds + lay.bits);
+    const bool lane_ok = l.f < F;
     CheckState* st_base = nullptr;
     float* Rm = nullptr;
     if (SP) {
         Rm = static_cast<float*>(ws) + static_cast<size_t>(l.live ? l.frame : 0) * c.num_edges;
     } else if (STATE_LDS) {
-        st_base = reinterpret_cast<CheckState*>(lds + state_offset(F, N)) +
+        st_base = reinterpret_cast<CheckState*>(lds + lay.state) +
                   static_cast<size_t>(l.f < F ? l.f : 0) * c.num_checks;
     } else {
         st_base = static_cast<CheckState*>(ws) +
@@ -82,15 +88,23 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
     bool done = !l.live;
     bool ok = false;
     int iters = 0;
-    if (prm.early_stop) {
-        ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
-        done = done || ok;
-    }
     int it = 0;
     // The state of the next block row is loaded while the current one is
     // processed; before the first iteration every state is zero.
     CheckState st_next = zero_state();
-    while (__syncthreads_or(!done) && it < prm.max_iterations) {
+    for (;;) {
+        // The syndrome is checked before the first iteration and after each
+        // one with early stop, else once after the last (one call site keeps
+        // the kernel's registers down).
+        const bool last = it >= prm.max_iterations;
+        if (prm.early_stop || last) {
+            const bool now = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, !done, F, s_flag);
+            if (!done) {
+                ok = now;
+                if (prm.early_stop) done = now;
+            }
+        }
+        if (last || !__syncthreads_or(!done)) break;
         const bool first = it == 0;
         for (int br = 0; br < mb; ++br) {
             if (!done) {
@@ -104,7 +118,8 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
                     } else if (mb > 1) {
                         st_next = st_base[l.r];  // block row 0, stored this iteration
                     }
-                    const CheckState ns = qc_ms_row<P, true>(c, s_tab, a, st, br, l.r, l.p, prm);
+                    const CheckState ns =
+                        qc_ms_row<P, true, SHORT ? kShortRow : kMaxDegree / P>(c, s_tab, a, st, br, l.r, l.p, prm);
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
                 }
@@ -113,15 +128,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_layered(
         }
         ++it;
         if (!done) iters = it;
-        if (prm.early_stop) {
-            const bool now = qc_syndrome_ok<P>(c, s_tab, a, l, !done, F, s_flag);
-            if (!done) {
-                ok = now;
-                done = now;
-            }
-        }
     }
-    if (!prm.early_stop) ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
     __syncthreads();
     if (l.live)
         store_frame(c, a, l.frame, l.u, Z * P, iters, ok, app_out, bits_out, iters_out, valid_out);
@@ -143,6 +150,8 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
     __syncthreads();
 
+    unsigned* bits = reinterpret_cast<unsigned*>(lds + qc_lds(F, N, Z, c.num_checks, false).bits);
+    const bool lane_ok = l.f < F;
     const size_t fr = static_cast<size_t>(l.live ? l.frame : 0);
     CheckState* st_base = SP ? nullptr : static_cast<CheckState*>(ws) + fr * c.num_checks;
     float* Rm = SP ? static_cast<float*>(ws) + fr * c.num_edges : nullptr;
@@ -151,12 +160,18 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     bool done = !l.live;
     bool ok = false;
     int iters = 0;
-    if (prm.early_stop) {
-        ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
-        done = done || ok;
-    }
     int it = 0;
-    while (__syncthreads_or(!done) && it < prm.max_iterations) {
+    for (;;) {
+        // Syndrome checks as in ldpc_qc_layered.
+        const bool last = it >= prm.max_iterations;
+        if (prm.early_stop || last) {
+            const bool now = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, !done, F, s_flag);
+            if (!done) {
+                ok = now;
+                if (prm.early_stop) done = now;
+            }
+        }
+        if (last || !__syncthreads_or(!done)) break;
         const bool first = it == 0;
         // Check phase: every row reads the previous iteration's APP.
         if (!done) {
@@ -198,15 +213,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         __syncthreads();
         ++it;
         if (!done) iters = it;
-        if (prm.early_stop) {
-            const bool now = qc_syndrome_ok<P>(c, s_tab, a, l, !done, F, s_flag);
-            if (!done) {
-                ok = now;
-                done = now;
-            }
-        }
     }
-    if (!prm.early_stop) ok = qc_syndrome_ok<P>(c, s_tab, a, l, l.live, F, s_flag);
     __syncthreads();
     if (l.live)
         store_frame(c, a, l.frame, l.u, Z * P, iters, ok, app_out, bits_out, iters_out, valid_out);
@@ -335,6 +342,7 @@ struct Plan {
     int F;             // QC frames per workgroup
     int P;             // QC lanes per row
     bool state_lds;    // QC layered min-sum
+    bool short_rows;   // QC layered min-sum, P = 1, rows of at most kShortRow edges
     int threads;
     size_t lds_bytes;
     size_t ws_per_frame;  // bytes
@@ -412,13 +420,17 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
     const int N = c->num_variables;
     const int Z = c->lifting;
     const size_t app_bytes = static_cast<size_t>(N) * 4;
-    bool qc_ok = Z > 0 && Z <= kMaxQcThreads && app_bytes <= lds_max &&
-                 c->max_row_degree <= kMaxDegree && c->num_slots <= kMaxSlots;
+    // APP plus sign bitmaps of one frame.
+    const size_t frame_bytes =
+        Z > 0 ? qc_lds(1, N, Z, c->num_checks, false).end * 4 : app_bytes;
+    bool qc_ok = Z > 0 && Z <= kMaxQcThreads && frame_bytes <= lds_max &&
+                 c->max_row_degree <= kMaxDegree && c->num_slots <= kMaxSlots &&
+                 syn_words(Z, N) < (1u << 21);
     if (m->kernel == 2 && !qc_ok)
         return ck::fail(CK_EINVAL,
                         "quasi-cyclic kernel requested but the code has no usable QC view "
-                        "(lifting %d, max row degree %d, %d slots, APP %zu bytes, LDS %zu bytes)",
-                        Z, c->max_row_degree, c->num_slots, app_bytes, lds_max);
+                        "(lifting %d, max row degree %d, %d slots, %zu bytes per frame, LDS %zu bytes)",
+                        Z, c->max_row_degree, c->num_slots, frame_bytes, lds_max);
     if (m->kernel == 1) qc_ok = false;
     if (qc_ok) {
         p.kind = 2;
@@ -429,13 +441,12 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
         int F = m->frames_per_group;
         if (F == 0) F = std::max(1, 256 / (Z * P));
         F = std::min(F, kMaxQcThreads / (Z * P));
-        F = std::min<int>(F, static_cast<int>(lds_max / app_bytes));
-        F = std::max(F, 1);
+        while (F > 1 && qc_lds(F, N, Z, c->num_checks, false).end * 4 > lds_max) --F;
         const size_t state_bytes = static_cast<size_t>(c->num_checks) * sizeof(CheckState);
         bool state_lds = false;
         if (!p.sp && !p.flood) {
-            const size_t with = state_offset(F, N) * 4 + static_cast<size_t>(F) * state_bytes;
-            const size_t without = static_cast<size_t>(F) * app_bytes;
+            const size_t with = qc_lds(F, N, Z, c->num_checks, true).end * 4;
+            const size_t without = qc_lds(F, N, Z, c->num_checks, false).end * 4;
             if (m->state_in_lds == 1) {
                 if (with > lds_max)
                     return ck::fail(CK_EINVAL, "check state does not fit in LDS (%zu > %zu bytes)",
@@ -449,9 +460,9 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
         p.F = F;
         p.P = P;
         p.state_lds = state_lds;
+        p.short_rows = !p.sp && !p.flood && P == 1 && c->max_row_degree <= kShortRow;
         p.threads = static_cast<int>(round_up(static_cast<size_t>(F) * Z * P, 64));
-        p.lds_bytes = state_lds ? state_offset(F, N) * 4 + static_cast<size_t>(F) * state_bytes
-                                : static_cast<size_t>(F) * app_bytes;
+        p.lds_bytes = qc_lds(F, N, Z, c->num_checks, state_lds).end * 4;
         if (p.sp)
             p.ws_per_frame = static_cast<size_t>(c->num_edges) * 4;
         else
@@ -479,11 +490,11 @@ void allow_lds(K kernel, size_t bytes) {
     }
 }
 
-template <bool SP, int P, bool SL>
+template <bool SP, int P, bool SL, bool SHORT = false>
 void launch_qc_layered(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
                        signed char* bits, int* iters, unsigned char* valid, void* ws, long long R,
                        const Params& prm, hipStream_t s) {
-    auto kernel = ldpc_qc_layered<SP, P, SL>;
+    auto kernel = ldpc_qc_layered<SP, P, SL, SHORT>;
     allow_lds(kernel, p.lds_bytes);
     const long long groups = (R + p.F - 1) / p.F;
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(groups)), dim3(p.threads), p.lds_bytes,
@@ -554,11 +565,13 @@ extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method
         } else if (p.sp) {
             launch_qc_layered<true, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else if (p.state_lds) {
-            if (p.P == 1) launch_qc_layered<false, 1, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows) launch_qc_layered<false, 1, true, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.P == 1) launch_qc_layered<false, 1, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 2) launch_qc_layered<false, 2, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_layered<false, 4, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else {
-            if (p.P == 1) launch_qc_layered<false, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows) launch_qc_layered<false, 1, false, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.P == 1) launch_qc_layered<false, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 2) launch_qc_layered<false, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_layered<false, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         }

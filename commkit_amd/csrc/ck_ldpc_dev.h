@@ -10,7 +10,7 @@
 namespace ckdev {
 
 constexpr int kGenericThreads = 256;
-constexpr int kMaxQcThreads = 1024;
+constexpr int kMaxQcThreads = 768;  // 12 waves: up to 168 VGPRs per lane
 constexpr int kMaxFrames = 64;   // frames per QC workgroup
 constexpr int kMaxDegree = 32;   // check degree of the compressed min-sum state
 constexpr int kMaxSlots = 480;   // nonzero circulants of a QC code the QC kernels take
@@ -41,9 +41,30 @@ __device__ __forceinline__ float phi(float x) {
     return -logf(tanhf(0.5f * x));
 }
 
-// Offset (floats) of the LDS check states after F APPs of N floats, 16-byte aligned.
-__host__ __device__ __forceinline__ size_t state_offset(int F, int N) {
-    return (static_cast<size_t>(F) * N + 3) & ~static_cast<size_t>(3);
+// Sign bitmaps of the syndrome check: per frame and block column j, bit c
+// is (APP of variable jZ + (c mod Z)) < 0 for c in [0, 64 ceil((Z + 32) / 64)),
+// so the 32 rows from any row below Z read two adjacent words, unwrapped.
+__host__ __device__ __forceinline__ int syn_words_per_col(int Z) { return 2 * ((Z + 95) / 64); }
+__host__ __device__ __forceinline__ size_t syn_words(int Z, int N) {
+    return static_cast<size_t>(N / Z) * syn_words_per_col(Z);
+}
+
+__host__ __device__ __forceinline__ size_t align4(size_t x) { return (x + 3) & ~static_cast<size_t>(3); }
+
+// Dynamic LDS of the QC kernels, in floats: F APPs of N floats, then F frames
+// of sign bitmaps, then (layered min-sum with the state in LDS) F frames of
+// m check states; each part 16-byte aligned.
+struct QcLds {
+    size_t bits;
+    size_t state;
+    size_t end;
+};
+__host__ __device__ __forceinline__ QcLds qc_lds(int F, int N, int Z, int m, bool state) {
+    QcLds o;
+    o.bits = align4(static_cast<size_t>(F) * N);
+    o.state = align4(o.bits + static_cast<size_t>(F) * syn_words(Z, N));
+    o.end = state ? o.state + static_cast<size_t>(F) * m * 4 : o.state;
+    return o;
 }
 
 __device__ __forceinline__ float state_msg(const CheckState& st, int e) {
@@ -137,22 +158,32 @@ __device__ __forceinline__ T ld_uniform(const T* p, int i) {
 // holds {block column * Z, shift} per nonzero circulant (S + 32 entries, zero
 // padded); lane r of a block row reaches variable jZ + (r + shift) mod Z of
 // slot {jZ, shift}, which in bytes from &a[r] is
-//   4 (jZ + shift) - (r >= Z - shift ? 4Z : 0),
-// so the LDS copy holds {4 (jZ + shift), Z - shift}.  Padding {0, 0} becomes
-// {0, Z}: offset 0, the lane's own row, inside the frame.
+//   4 (jZ + shift) - (r >= Z - shift ? 4Z : 0).
+// The LDS copy holds x = 4 (jZ + shift) and y = (Z - shift) << 21 | jW, where
+// jW = j * syn_words_per_col(Z) locates the column's sign bitmap.  With
+// rr = r << 21 | (2^21 - 1), r >= Z - shift is rr >= y (unsigned).  Padding
+// {0, 0} becomes x = 0, y = Z << 21: offset 0, the lane's own row.
 __device__ inline void stage_slot_table(const ck_ldpc_code& c, int2* tab) {
     const int n = c.num_slots + kMaxDegree;
     const int Z = c.lifting;
+    const int W = syn_words_per_col(Z);
     const int2* src = reinterpret_cast<const int2*>(c.qc_slot_packed);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int2 v = src[i];
-        tab[i] = make_int2(4 * (v.x + v.y), Z - v.y);
+        const unsigned y = (static_cast<unsigned>(Z - v.y) << 21) |
+                           static_cast<unsigned>((v.x / Z) * W);
+        tab[i] = make_int2(4 * (v.x + v.y), static_cast<int>(y));
     }
 }
 
-// Byte offset from &a[r] of lane r's variable in the staged slot `v`.
-__device__ __forceinline__ int qc_off(int2 v, int r, int z4) {
-    return v.x + ((r >= v.y) ? -z4 : 0);
+__device__ __forceinline__ unsigned row_rank(int r) {
+    return (static_cast<unsigned>(r) << 21) | 0x1fffffu;
+}
+
+// Byte offset from &a[r] of lane r's variable in the staged slot `v`
+// (rr = row_rank(r)).
+__device__ __forceinline__ int qc_off(int2 v, unsigned rr, int z4) {
+    return v.x + ((rr >= static_cast<unsigned>(v.y)) ? -z4 : 0);
 }
 
 __device__ __forceinline__ float& lds_at(float* ar, int off) {
@@ -164,40 +195,122 @@ __device__ __forceinline__ float with_sign(float mag, unsigned bits, int e) {
     return __uint_as_float(__float_as_uint(mag) | ((bits << (31 - e)) & 0x80000000u));
 }
 
-// Whether every check of the lane's frame is satisfied; `test` says whether
-// this lane's frame is being checked.  Called by every lane (barriers).
-template <int P>
-__device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const float* a,
-                               const QcLane& l, bool test, int F, int* s_flag) {
-    const int z4 = 4 * c.lifting;
-    float* ar = const_cast<float*>(a) + l.r;
-    int bad = 0;
-    if (test) {
-        // Lane p of a row checks the rows of block rows p, p+P, ...
-        for (int br0 = 0; br0 < c.base_rows; br0 += P) {
-            const int br = min(br0 + l.p, c.base_rows - 1);
-            const int s0 = ld_uniform(c.qc_row_ptr, br);
-            const int s1 = ld_uniform(c.qc_row_ptr, br + 1);
-            unsigned parity = 0u;
-            for (int s = s0; s < s1; s += 8) {
-                float v[8];
+// Whether every check of the lane's frame slot is satisfied.  `bits` is the
+// sign-bitmap area of frame slot 0 (slot f at bits + f * syn_words), `app`
+// the APP of slot 0 (slot f at app + f * N); `test` (the same on every lane of
+// a slot) says whether the slot is checked.  Called by every lane (barriers);
+// lanes outside the F slots (lane_ok false) return true.
+//
+// The waves first write the bitmaps, one ballot of 64 signs per step; then
+// each task (slot, block row, 32 rows from r0) XORs, over the block row's
+// circulants {j, shift}, the 32 bits from column (r0 + shift) mod Z of
+// bitmap j: bit i of the result is the parity of row r0 + i.
+__device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const float* app,
+                               unsigned* bits, const QcLane& l, bool lane_ok, bool test, int F,
+                               int* s_flag) {
+    constexpr int U = 8;  // loads in flight per lane
+    const int Z = c.lifting;
+    const int N = c.num_variables;
+    const int nb = N / Z;
+    const int W = syn_words_per_col(Z);
+    const int C = W >> 1;
+    const int words = nb * W;
+    const int K = (Z + 31) >> 5;
+    const int per_frame = c.base_rows * K;
+    const int checks = F * per_frame;
+    if (lane_ok && l.u == 0) s_flag[l.f] = test ? 0 : 2;
+    // The first check task of this lane; its row range is fetched now so the
+    // load completes under the bitmap writes.
+    int task = threadIdx.x;
+    int s0 = 0;
+    int s1 = 0;
+    if (task < checks) {
+        const int br = (task % per_frame) / K;
+        s0 = c.qc_row_ptr[br];
+        s1 = c.qc_row_ptr[br + 1];
+    }
+    {
+        // Chunk t of the F * nb * C chunks (slot f, column j, bits 64 ch ..
+        // 64 ch + 63, t = (f nb + j) C + ch) goes to words 2t, 2t + 1; lane i
+        // of its wave reads variable (64 ch + i) mod Z of column f nb + j,
+        // whose APP starts at app + (f nb + j) Z.  Wave w writes chunks
+        // [t0, t1); a group of up to eight chunks is stored by lanes 0..7.
+        const int lane = threadIdx.x & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+        const int nw = static_cast<int>(blockDim.x >> 6);
+        const int T = F * nb * C;
+        const int tpw = (T + nw - 1) / nw;
+        const int t0 = min(T, wave * tpw);
+        const int t1 = min(T, t0 + tpw);
+        int fj = t0 / C;
+        int ch = t0 - fj * C;
+        const float* src = app + static_cast<size_t>(fj) * Z;
+        const int d = 64 % Z;   // column step from one chunk to the next
+        const int lm = lane % Z;
+        int col = (ch * 64 + lane) % Z;
+        for (int t = t0; t < t1; t += U) {
+            float v[U];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int2 e = tab[s + u];  // padded: s + u < S + 32
-                    v[u] = lds_at(ar, qc_off(e, l.r, z4));
+            for (int u = 0; u < U; ++u) {
+                v[u] = *((t + u < t1) ? src + col : app);  // unconditional load
+                if (++ch == C) {
+                    ch = 0;
+                    src += Z;
+                    col = lm;
+                } else {
+                    col += d;
+                    col -= (col >= Z) ? Z : 0;
                 }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) parity ^= (s + u < s1 && v[u] < 0.0f) ? 1u : 0u;
             }
-            bad |= (br0 + l.p < c.base_rows) ? static_cast<int>(parity) : 0;
+            unsigned lo = 0u;
+            unsigned hi = 0u;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned long long m = __ballot(v[u] < 0.0f);
+                lo = (lane == u) ? static_cast<unsigned>(m) : lo;
+                hi = (lane == u) ? static_cast<unsigned>(m >> 32) : hi;
+            }
+            if (lane < U && t + lane < t1)
+                *reinterpret_cast<uint2*>(bits + 2 * static_cast<size_t>(t + lane)) = make_uint2(lo, hi);
         }
     }
-    if (F == 1) return !__syncthreads_or(bad);
-    if (threadIdx.x < F) s_flag[threadIdx.x] = 0;
     __syncthreads();
-    if (bad) s_flag[l.f] = 1;
+    for (; task < checks; task += blockDim.x) {
+        const int f = task / per_frame;
+        const int rem = task - f * per_frame;
+        const int br = rem / K;
+        const int r0 = (rem - br * K) << 5;
+        if (task >= static_cast<int>(blockDim.x)) {
+            s0 = c.qc_row_ptr[br];
+            s1 = c.qc_row_ptr[br + 1];
+        }
+        if (s_flag[f] != 0) continue;  // not checked, or already failed
+        const unsigned* bm = bits + static_cast<size_t>(f) * words;
+        unsigned acc = 0u;
+        for (int s = s0; s < s1; s += U) {
+            // Entries past the row's end (other rows or padding) stay in bounds.
+            unsigned lo[U];
+            unsigned hi[U];
+            int sh[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned y = static_cast<unsigned>(tab[s + u].y);
+                const int d = r0 - static_cast<int>(y >> 21);  // r0 - (Z - shift)
+                const int o = d + ((d < 0) ? Z : 0);            // (r0 + shift) mod Z
+                const unsigned* w = bm + (y & 0x1fffffu) + (o >> 5);
+                lo[u] = w[0];
+                hi[u] = w[1];
+                sh[u] = o & 31;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (s + u < s1) acc ^= __builtin_amdgcn_alignbit(hi[u], lo[u], static_cast<unsigned>(sh[u]));
+        }
+        if (Z - r0 < 32) acc &= (1u << (Z - r0)) - 1u;
+        if (acc != 0u) s_flag[f] = 1;
+    }
     __syncthreads();
-    const bool ok = (l.f < F) ? (s_flag[l.f] == 0) : true;
+    const bool ok = lane_ok ? (s_flag[l.f] == 0) : true;
     __syncthreads();
     return ok;
 }
@@ -226,9 +339,10 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
     float* addr[NT];
     float* ar = a + r;
     const int z4 = 4 * Z;
+    const unsigned rr = row_rank(r);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        addr[t] = &lds_at(ar, qc_off(tab[s0 + t * P + p], r, z4));
+        addr[t] = &lds_at(ar, qc_off(tab[s0 + t * P + p], rr, z4));
         q[t] = *addr[t];
     }
     // Pass 1.  min1 / min2 / idx follow the sequential rule exactly:
@@ -292,27 +406,32 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
 
 // Chooses the instantiation of qc_ms_row_n by the row's slot count, rounded
 // up to a multiple of four (the count is uniform across the workgroup).
-template <int P, bool UPDATE_APP, int G = 1>
+template <int P, bool UPDATE_APP, int MAXT, int G = 1>
 __device__ __forceinline__ CheckState qc_ms_dispatch(int groups, const int2* tab, float* a,
                                                      const CheckState st, int s0, int dc, int r,
                                                      int p, int Z, const Params& prm) {
-    if constexpr (4 * G * P >= kMaxDegree) {
+    if constexpr (4 * G >= MAXT) {
         return qc_ms_row_n<P, 4 * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
     } else {
         if (groups <= G) return qc_ms_row_n<P, 4 * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
-        return qc_ms_dispatch<P, UPDATE_APP, G + 1>(groups, tab, a, st, s0, dc, r, p, Z, prm);
+        return qc_ms_dispatch<P, UPDATE_APP, MAXT, G + 1>(groups, tab, a, st, s0, dc, r, p, Z, prm);
     }
 }
 
-// One min-sum check row `r` of block row `br`.
-template <int P, bool UPDATE_APP>
+// Slots per lane of the kernels for rows of up to kShortRow edges (P = 1):
+// fewer registers, so two workgroups of six waves share a CU.
+constexpr int kShortRow = 24;
+
+// One min-sum check row `r` of block row `br`; MAXT bounds the lane's slots
+// (ceil(row degree / P)), which the host checks before choosing the kernel.
+template <int P, bool UPDATE_APP, int MAXT = kMaxDegree / P>
 __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int2* tab, float* a,
                                                 const CheckState st, int br, int r, int p,
                                                 const Params& prm) {
     const int s0 = ld_uniform(c.qc_row_ptr, br);
     const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
     const int groups = ((dc + P - 1) / P + 3) >> 2;
-    return qc_ms_dispatch<P, UPDATE_APP>(groups, tab, a, st, s0, dc, r, p, c.lifting, prm);
+    return qc_ms_dispatch<P, UPDATE_APP, MAXT>(groups, tab, a, st, s0, dc, r, p, c.lifting, prm);
 }
 
 // One sum-product check row; Rm is the frame's messages, [slot][Z].  The old
@@ -336,10 +455,11 @@ __device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab
     unsigned neg = 0u;
     float* ar = a + r;
     const int z4 = 4 * Z;
+    const unsigned rr = row_rank(r);
 #pragma unroll
     for (int t = 0; t < MAXT; ++t) {
         if (t < dc) {
-            addr[t] = qc_off(tab[s0 + t], r, z4);
+            addr[t] = qc_off(tab[s0 + t], rr, z4);
             q[t] = lds_at(ar, addr[t]) - rold[t];
             neg ^= (q[t] < 0.0f) ? 1u : 0u;
             ph[t] = phi(fabsf(q[t]));
