@@ -50,11 +50,12 @@ namespace {
 
 using namespace ckdev;
 
-// SHORT (min-sum, one lane per row, rows of at most kShortRow edges): at
-// most 128 VGPRs, so that two workgroups of up to six waves each fit a CU.
+// SHORT (min-sum, rows of at most kShortRow edges): at most 128 VGPRs for
+// P = 1 (two workgroups of up to six waves per CU), 80 for P = 2 (two of up
+// to twelve).
 template <bool SP, int P, bool STATE_LDS, bool SHORT>
 __global__ __launch_bounds__(kMaxQcThreads)
-__attribute__((amdgpu_waves_per_eu(SHORT ? 4 : 1))) void ldpc_qc_layered(
+__attribute__((amdgpu_waves_per_eu(SHORT ? 4 * P - (P > 1 ? 2 : 0) : 1))) void ldpc_qc_layered(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm) {
@@ -119,7 +120,7 @@ __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : 1))) void ldpc_qc_layered(
                         st_next = st_base[l.r];  // block row 0, stored this iteration
                     }
                     const CheckState ns =
-                        qc_ms_row<P, true, SHORT ? kShortRow : kMaxDegree / P>(c, s_tab, a, st, br, l.r, l.p, prm);
+                        qc_ms_row<P, true, SHORT ? kShortRow / P : kMaxDegree / P>(c, s_tab, a, st, br, l.r, l.p, prm);
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
                 }
@@ -405,7 +406,7 @@ namespace {
 using ck::check_code;
 using ck::check_method;
 
-int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
+int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, Plan* plan) {
     if (int32_t st = check_code(c)) return st;
     if (int32_t st = check_method(m)) return st;
     ck::DeviceInfo info{};
@@ -439,7 +440,12 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
         if (p.sp) P = 1;
         if (Z * P > kMaxQcThreads) P = 1;
         int F = m->frames_per_group;
-        if (F == 0) F = std::max(1, 256 / (Z * P));
+        if (F == 0) {
+            // Frames share a workgroup once there are two workgroups per CU
+            // without sharing; more lanes per workgroup, fewer per-row barriers.
+            const long long per = R / (2LL * info.num_cus);
+            F = static_cast<int>(std::max(1LL, std::min<long long>(per, kMaxFrames)));
+        }
         F = std::min(F, kMaxQcThreads / (Z * P));
         while (F > 1 && qc_lds(F, N, Z, c->num_checks, false).end * 4 > lds_max) --F;
         const size_t state_bytes = static_cast<size_t>(c->num_checks) * sizeof(CheckState);
@@ -460,7 +466,7 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, Plan* plan) {
         p.F = F;
         p.P = P;
         p.state_lds = state_lds;
-        p.short_rows = !p.sp && !p.flood && P == 1 && c->max_row_degree <= kShortRow;
+        p.short_rows = !p.sp && !p.flood && P <= 2 && c->max_row_degree <= kShortRow;
         p.threads = static_cast<int>(round_up(static_cast<size_t>(F) * Z * P, 64));
         p.lds_bytes = qc_lds(F, N, Z, c->num_checks, state_lds).end * 4;
         if (p.sp)
@@ -529,7 +535,7 @@ void launch_generic(const Plan& p, const ck_ldpc_code& c, const float* llr, floa
 extern "C" int64_t ck_ldpc_workspace_bytes(const ck_ldpc_code* code, const ck_ldpc_method* method,
                                            int64_t R) {
     Plan p{};
-    if (make_plan(code, method, &p) != CK_OK || R < 0) return -1;
+    if (R < 0 || make_plan(code, method, R, &p) != CK_OK) return -1;
     return static_cast<int64_t>(round_up(p.ws_per_frame * static_cast<size_t>(R), 256));
 }
 
@@ -538,7 +544,7 @@ extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method
                                   int32_t* iterations, uint8_t* valid, int64_t R, void* workspace,
                                   void* stream) {
     Plan p{};
-    if (int32_t st = make_plan(code, method, &p)) return st;
+    if (int32_t st = make_plan(code, method, R, &p)) return st;
     CK_REQUIRE(R >= 0, "R = %lld < 0", static_cast<long long>(R));
     if (R == 0) return CK_OK;
     CK_REQUIRE(R <= 0x7fffffffLL, "R = %lld exceeds one launch", static_cast<long long>(R));
@@ -565,13 +571,14 @@ extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method
         } else if (p.sp) {
             launch_qc_layered<true, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else if (p.state_lds) {
-            if (p.short_rows) launch_qc_layered<false, 1, true, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows && p.P == 1) launch_qc_layered<false, 1, true, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 1) launch_qc_layered<false, 1, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 2) launch_qc_layered<false, 2, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_layered<false, 4, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else {
-            if (p.short_rows) launch_qc_layered<false, 1, false, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows && p.P == 1) launch_qc_layered<false, 1, false, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 1) launch_qc_layered<false, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, false, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 2) launch_qc_layered<false, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_layered<false, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         }
