@@ -109,7 +109,7 @@ __device__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_co
 }
 
 template <bool SP, int P>
-__device__ void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s_tab, int* s_need,
+__device__ __attribute__((noinline)) void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s_tab, int* s_need,
                           int* s_frame, int* s_flag) {
     const BatchClass bc = d->cls[ci];
     const ck_ldpc_code c = bc.code;

@@ -265,10 +265,12 @@ __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float
 }
 
 // CLOSED: closed form (max-log, uniform); else level metrics.
+// inv_rows (optional): 1/sigma^2 per row of row_len symbols, instead of inv_s2.
 template <int KB, bool CLOSED>
 __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
     const float2* __restrict__ x, long long n, ck_axis re, ck_axis im, float inv_s2f,
-    double inv_s2, int exact, float* __restrict__ out) {
+    double inv_s2, int exact, float* __restrict__ out, const double* __restrict__ inv_rows,
+    long long row_len) {
     constexpr int K = 2 * KB;
     __shared__ AxisTab<KB> tab[2];
     __shared__ float tile[kThreads * (K | 1)];
@@ -283,6 +285,14 @@ __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
         const long long i = base + threadIdx.x;
         if (i < n) {
             const float2 v = x[i];
+            if (inv_rows != nullptr) {
+                const long long r = (n <= 0xffffffffLL)
+                                        ? static_cast<long long>(static_cast<unsigned>(i) /
+                                                                 static_cast<unsigned>(row_len))
+                                        : i / row_len;
+                inv_s2 = inv_rows[r];
+                inv_s2f = static_cast<float>(inv_s2);
+            }
             if (CLOSED) {
                 axis_closed<KB>(tab[0], v.x, inv_s2, inv_step0, row);
                 axis_closed<KB>(tab[1], v.y, inv_s2, inv_step1, row);
@@ -474,7 +484,8 @@ int32_t check_axis(const ck_axis* a, const char* name, bool closed) {
 
 template <bool CLOSED>
 int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_axis* im,
-                         double inv_s2, int exact, float* out, void* stream) {
+                         double inv_s2, int exact, float* out, void* stream,
+                         const double* inv_rows = nullptr, int64_t row_len = 1) {
     CK_REQUIRE(n >= 0, "n = %lld < 0", static_cast<long long>(n));
     if (int32_t st = check_axis(re, "re", CLOSED)) return st;
     if (int32_t st = check_axis(im, "im", CLOSED)) return st;
@@ -492,7 +503,8 @@ int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_a
     case KB:                                                                                  \
         hipLaunchKernelGGL((llr_separable_kernel<KB, CLOSED>), dim3(grid), dim3(kThreads), 0, \
                            s, xs, static_cast<long long>(n), *re, *im,                        \
-                           static_cast<float>(inv_s2), inv_s2, exact, out);                   \
+                           static_cast<float>(inv_s2), inv_s2, exact, out, inv_rows,          \
+                           static_cast<long long>(row_len));                                  \
         break;
         CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
 #undef CK_CASE
@@ -513,6 +525,25 @@ extern "C" int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
                                     void* stream) {
     CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
     return launch_separable<false>(x, n, re, im, inv_s2, exact, out, stream);
+}
+
+extern "C" int32_t ck_llr_separable_rows(const void* x, int64_t rows, int64_t row_len,
+                                         const ck_axis* re, const ck_axis* im,
+                                         const double* inv_s2, int32_t closed, int32_t exact,
+                                         float* out, void* stream) {
+    CK_REQUIRE(rows >= 0 && row_len >= 1, "rows %lld, row_len %lld", static_cast<long long>(rows),
+               static_cast<long long>(row_len));
+    CK_REQUIRE(closed == 0 || closed == 1, "closed must be 0 or 1, got %d", closed);
+    CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
+    CK_REQUIRE(!(closed && exact), "the closed form is max-log only");
+    CK_REQUIRE(rows <= INT64_MAX / row_len, "rows x row_len overflows");
+    if (rows == 0) return CK_OK;
+    CK_REQUIRE(inv_s2 != nullptr, "NULL inv_s2");
+    // 1 is a placeholder for the scalar check; the kernel reads inv_s2[row].
+    return closed ? launch_separable<true>(x, rows * row_len, re, im, 1.0, 0, out, stream, inv_s2,
+                                           row_len)
+                  : launch_separable<false>(x, rows * row_len, re, im, 1.0, exact, out, stream,
+                                            inv_s2, row_len);
 }
 
 extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,

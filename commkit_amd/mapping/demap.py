@@ -206,3 +206,46 @@ def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
                     out.data_ptr(), stream,
                 ))  # fmt: skip
     return out.reshape(*shape[:-1], shape[-1] * k) if len(shape) else out.reshape(-1)
+
+
+def llr_rows(c: "Constellation", x: Any, noise_var: Any, method: str, out: Any = None) -> Any:
+    """LLRs of ``(R, S)`` complex device symbols, row ``r`` with noise variance ``noise_var[r]``.
+
+    The rows of frames at mixed Es/N0 in one launch; row ``r`` equals
+    ``llr_any(c, x[r], noise_var[r], method)`` bit for bit.  Separable
+    constellations only.  ``out``: optional ``(R, S * k)`` float32 contiguous
+    device tensor to write into.  No counterpart in the reference (its
+    ``compute_llr`` takes one noise variance, ``commkit/mapping/llr.py:24-95``).
+    """
+    import torch
+
+    check_method(method)
+    if c._axes is None:
+        raise ValueError("llr_rows: the constellation is not separable per axis")
+    if x.dim() != 2 or not x.is_complex():
+        raise ValueError("llr_rows: symbols must be a (rows, symbols) complex tensor")
+    R, S = int(x.shape[0]), int(x.shape[1])
+    nv = np.asarray(noise_var, dtype=np.float64).reshape(-1)
+    if nv.shape[0] != R:
+        raise ValueError(f"llr_rows: {nv.shape[0]} noise variances for {R} rows")
+    if not np.all(np.isfinite(nv)) or np.any(nv < 0):
+        raise ValueError("llr_rows: noise variances must be finite and >= 0")
+    inv = torch.from_numpy(1.0 / np.maximum(nv, 1e-20)).to(x.device)
+    k = c.bits_per_symbol
+    flat = x.to(torch.complex64).contiguous()
+    if out is None:
+        out = torch.empty((R, S * k), dtype=torch.float32, device=x.device)
+    elif (out.shape != (R, S * k) or out.dtype != torch.float32 or not out.is_contiguous()
+          or out.device != x.device):
+        raise ValueError(f"llr_rows: out must be a contiguous ({R}, {S * k}) float32 tensor")
+    if R * S == 0:
+        return out
+    lib = _native.lib()
+    with torch.cuda.device(x.device):
+        re_ax, im_ax, _keep = axis_tables(c, x.device)
+        closed = closed_form(c._axes, method)
+        _native.check(lib.ck_llr_separable_rows(
+            flat.data_ptr(), R, S, ctypes.byref(re_ax), ctypes.byref(im_ax), inv.data_ptr(),
+            int(closed), int(method == "exact"), out.data_ptr(), _native.stream_of(x.device),
+        ))  # fmt: skip
+    return out

@@ -30,7 +30,18 @@ __all__ = ["WATERFALL_DB", "waterfall_db", "calibrate", "frame_error_rate"]
 #: ``python -m commkit_amd.operating`` on an MI355X.
 WATERFALL_DB: dict[tuple[str, int], float] = {
     ("r4/5-z360", 64): 16.65,
+    ("r1/2-z27", 4): 3.35,
     ("r1/2-z27", 16): 9.2,
+    ("r1/2-z27", 64): 14.15,
+    ("r1/2-z27", 256): 19.0,
+    ("r2/3-z96", 4): 3.75,
+    ("r2/3-z96", 16): 9.65,
+    ("r2/3-z96", 64): 14.85,
+    ("r2/3-z96", 256): 19.8,
+    ("r5/6-z360", 4): 5.45,
+    ("r5/6-z360", 16): 11.75,
+    ("r5/6-z360", 64): 17.25,
+    ("r5/6-z360", 256): 22.65,
 }
 
 _GRID = 0.05

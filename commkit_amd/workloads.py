@@ -32,7 +32,8 @@ import numpy as np
 from .coding.ldpc import LDPCCode
 from .mapping.constellation import Constellation
 
-__all__ = ["CODES", "seeded_base", "seeded_code", "Frames", "make_frames", "code_by_name"]
+__all__ = ["CODES", "seeded_base", "seeded_code", "Frames", "make_frames", "code_by_name",
+           "MIXED_CODES", "MIXED_ORDERS", "Mixed", "MixedGroup", "make_mixed"]
 
 
 @dataclass(frozen=True)
@@ -182,3 +183,64 @@ def make_frames(code: LDPCCode, constellation: Constellation, esn0_db: Any, num_
     )
     rx = (tx + noise * std).to(torch.complex64)
     return Frames(info, words, tx, rx, es, 10.0 ** (-es / 10.0))
+
+
+#: Codes and constellation orders of the mixed workload.
+MIXED_CODES = ("r5/6-z360", "r2/3-z96", "r1/2-z27")
+MIXED_ORDERS = (4, 16, 64, 256)
+
+
+@dataclass
+class MixedGroup:
+    """Frames of one (code, constellation) pair of a mixed workload: rows
+    ``[row0, row1)`` of their code's class."""
+
+    code_index: int
+    order: int
+    row0: int
+    row1: int
+    frames: Frames
+
+
+@dataclass
+class Mixed:
+    """A mixed workload: per code (``MIXED_CODES`` order) its frame count,
+    and the (code, constellation) groups that fill its rows."""
+
+    codes: list
+    rows: list
+    groups: list
+
+    @property
+    def info_bits(self) -> int:
+        return sum(c.k * r for c, r in zip(self.codes, self.rows))
+
+
+def make_mixed(num_frames: int = 10_000, *, seed: int = 0, device: Any = None,
+               low_db: float = -0.5, high_db: float = 1.0) -> Mixed:
+    """Frames of uniformly drawn codes (``MIXED_CODES``) and Gray QAM orders
+    (``MIXED_ORDERS``), each at its own Es/N0 uniform in ``[wf + low_db,
+    wf + high_db]`` around the pair's measured waterfall.  Within a code the
+    frames are ordered by constellation."""
+    from .operating import waterfall_db
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ci = rng.integers(0, len(MIXED_CODES), size=num_frames)
+    oi = rng.integers(0, len(MIXED_ORDERS), size=num_frames)
+    off = rng.uniform(low_db, high_db, size=num_frames)
+    codes = [seeded_code(n) for n in MIXED_CODES]
+    rows = [int((ci == i).sum()) for i in range(len(codes))]
+    groups = []
+    for i, name in enumerate(MIXED_CODES):
+        r0 = 0
+        for j, order in enumerate(MIXED_ORDERS):
+            sel = (ci == i) & (oi == j)
+            cnt = int(sel.sum())
+            if cnt == 0:
+                continue
+            es = waterfall_db(name, order) + off[sel]
+            fr = make_frames(codes[i], Constellation.qam(order), es, cnt,
+                             seed=seed * 1000 + 10 * i + j + 1, device=device)  # fmt: skip
+            groups.append(MixedGroup(i, order, r0, r0 + cnt, fr))
+            r0 += cnt
+    return Mixed(codes, rows, groups)

@@ -82,6 +82,20 @@ int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
                          const ck_axis* im, double inv_s2, int32_t exact,
                          float* out, void* stream);
 
+/* Separable LLRs of `rows` rows of `row_len` symbols, each row with its own
+ * noise: inv_s2 is a (rows,) float64 device array of 1/sigma^2 (positive
+ * and finite; not checked on the device).  closed = 1: the closed form of
+ * ck_llr_separable_maxlog (exact must be 0); closed = 0: the level metrics
+ * of ck_llr_separable.  Row r gives the same LLRs, bit for bit, as the
+ * scalar entry point called on that row alone with inv_s2[r].
+ * No counterpart in the reference, whose compute_llr takes one noise
+ * variance per call (commkit/mapping/llr.py:24-95); this serves frames of
+ * mixed Es/N0 demapped in one launch. */
+int32_t ck_llr_separable_rows(const void* x, int64_t rows, int64_t row_len,
+                              const ck_axis* re, const ck_axis* im,
+                              const double* inv_s2, int32_t closed, int32_t exact,
+                              float* out, void* stream);
+
 /* LLRs from the metrics of all M points (any constellation).
  * points: (M,) complex128 device; log_prior: (M,) float64 device (zeros =
  * uniform); labels: (M,) int32 packed MSB-first labels; 2 <= M <= 65536,

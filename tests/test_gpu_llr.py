@@ -143,3 +143,36 @@ def test_real_pam(gpu):
     ref = llr_fp64(x, c.points, c.bit_labels, None, nv, "maxlog")
     ok, _ = llr_tolerance_ok(got, ref)
     assert ok
+
+
+@pytest.mark.parametrize("make,method", [
+    (lambda: Constellation.qam(4), "maxlog"),
+    (lambda: Constellation.qam(64), "maxlog"),
+    (lambda: Constellation.qam(16), "exact"),
+    (lambda: Constellation.qam(256).shaped(nu=0.05), "exact"),
+    (lambda: Constellation.qam(64).shaped(nu=0.05), "maxlog"),
+], ids=["qpsk-maxlog", "64qam-maxlog", "16qam-exact", "256qam-shaped-exact", "64qam-shaped-maxlog"])
+def test_llr_rows_equal_per_row_calls(gpu, make, method):
+    """Per-row noise (one launch) gives each row the LLRs of a scalar call, bit for bit."""
+    import torch
+
+    from commkit_amd.mapping.demap import llr_any, llr_rows
+
+    c = make()
+    rng = np.random.default_rng(11)
+    R, S = 9, 301
+    pts = c.points[rng.integers(0, c.order, size=(R, S))]
+    nv = rng.uniform(0.005, 0.4, size=R)
+    x = pts + np.sqrt(nv[:, None] / 2) * (rng.standard_normal((R, S)) + 1j * rng.standard_normal((R, S)))
+    xt = torch.from_numpy(x.astype(np.complex64)).cuda()
+    got = llr_rows(c, xt, nv, method)
+    out = torch.full((R, S * c.bits_per_symbol), float("nan"), device="cuda")
+    assert llr_rows(c, xt, nv, method, out=out) is out
+    for r in range(R):
+        want = llr_any(c, xt[r], float(nv[r]), method)
+        assert torch.equal(got[r], want), f"row {r}"
+        assert torch.equal(out[r], want), f"row {r} (out=)"
+    with pytest.raises(ValueError):
+        llr_rows(c, xt, nv[:-1], method)
+    with pytest.raises(ValueError):
+        llr_rows(Constellation.psk(8), xt, nv, method)
