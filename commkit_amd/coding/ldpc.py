@@ -569,8 +569,24 @@ def device_code(code: LDPCCode, device: Any) -> DeviceCode:
 
 
 #: Tile choices per (n, lifting, rule, schedule) written by the autotuner
-#: (commkit_amd.tune); absent keys use the library's defaults.
+#: (commkit_amd.tune, file tuned.json of the package); absent keys use the
+#: library's defaults.
 TUNED: dict[tuple[int, int, int, int], tuple[int, int, int, int]] = {}
+
+
+def _load_tuned() -> None:
+    import json
+    from pathlib import Path
+
+    path = Path(__file__).resolve().parent.parent / "tuned.json"
+    if not path.exists():
+        return
+    for e in json.loads(path.read_text()):
+        TUNED[(int(e["n"]), int(e["lifting"]), int(e["rule"]), int(e["schedule"]))] = tuple(
+            int(v) for v in e["tiles"])  # type: ignore[assignment]
+
+
+_load_tuned()
 
 
 def method_struct(code: LDPCCode, method: MinSum | SumProduct) -> CkLdpcMethod:

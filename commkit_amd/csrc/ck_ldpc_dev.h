@@ -190,6 +190,14 @@ __device__ __forceinline__ float& lds_at(float* ar, int off) {
     return *reinterpret_cast<float*>(reinterpret_cast<char*>(ar) + off);
 }
 
+// The value of lane (this lane ^ off), off = 1 or 2, within a quad: a DPP
+// move (one VALU op) instead of an LDS permute.  The partner lane (a lane of
+// the same check row) must be active.
+__device__ __forceinline__ int lane_xor(int v, int off) {
+    return off == 1 ? __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false)   // quad_perm 1,0,3,2
+                    : __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+}
+
 // `mag` (sign bit clear) with the sign of bit e of `bits`.
 __device__ __forceinline__ float with_sign(float mag, unsigned bits, int e) {
     return __uint_as_float(__float_as_uint(mag) | ((bits << (31 - e)) & 0x80000000u));
@@ -324,9 +332,14 @@ __device__ __forceinline__ CheckState zero_state() {
     return z;
 }
 
+// Slot granularity of the row instantiations: rows are padded to a multiple
+// of kSlotStep slots per lane (four for one lane per row, two for several).
+template <int P>
+constexpr int kSlotStep = P == 1 ? 4 : 2;
+
 // One min-sum check row of a block row over NT slots of the lane (edges
 // e = t*P + p), given its old state; returns the new state.  Branch-free:
-// every slot is processed, and only the last four can lie past the row's end
+// every slot is processed, and only the last kSlotStep can lie past the row's end
 // (their table entries are real or zero padding, so their addresses stay in
 // the frame).  UPDATE_APP: layered (APP = Q + R_new); else the flooding check
 // phase, which leaves the APP alone.  Q and the LDS address of every edge
@@ -358,7 +371,7 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
-        const bool valid = (t < NT - 4) || (e < dc);
+        const bool valid = (t < NT - kSlotStep<P>) || (e < dc);
         const float qv = q[t] - state_msg(st, e);
         q[t] = qv;
         const float mag = valid ? fabsf(qv) : ck::kInf;
@@ -369,10 +382,10 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
     }
 #pragma unroll
     for (int off = 1; off < P; off <<= 1) {
-        const float o1 = __shfl_xor(min1, off);
-        const float o2 = __shfl_xor(min2, off);
-        const int oi = __shfl_xor(idx, off);
-        qneg |= static_cast<unsigned>(__shfl_xor(static_cast<int>(qneg), off));
+        const float o1 = __int_as_float(lane_xor(__float_as_int(min1), off));
+        const float o2 = __int_as_float(lane_xor(__float_as_int(min2), off));
+        const int oi = lane_xor(idx, off);
+        qneg |= static_cast<unsigned>(lane_xor(static_cast<int>(qneg), off));
         if (o1 < min1 || (o1 == min1 && oi < idx)) {
             min2 = fminf(min1, o2);
             min1 = o1;
@@ -390,7 +403,7 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int e = t * P + p;
-            if ((t < NT - 4) || (e < dc)) {
+            if ((t < NT - kSlotStep<P>) || (e < dc)) {
                 const float mag = (e == idx) ? m2 : m1;
                 *addr[t] = q[t] + with_sign(mag, signs, e);
             }
@@ -405,15 +418,16 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
 }
 
 // Chooses the instantiation of qc_ms_row_n by the row's slot count, rounded
-// up to a multiple of four (the count is uniform across the workgroup).
+// up to a multiple of kSlotStep (the count is uniform across the workgroup).
 template <int P, bool UPDATE_APP, int MAXT, int G = 1>
 __device__ __forceinline__ CheckState qc_ms_dispatch(int groups, const int2* tab, float* a,
                                                      const CheckState st, int s0, int dc, int r,
                                                      int p, int Z, const Params& prm) {
-    if constexpr (4 * G >= MAXT) {
-        return qc_ms_row_n<P, 4 * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
+    constexpr int S = kSlotStep<P>;
+    if constexpr (S * G >= MAXT) {
+        return qc_ms_row_n<P, S * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
     } else {
-        if (groups <= G) return qc_ms_row_n<P, 4 * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
+        if (groups <= G) return qc_ms_row_n<P, S * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
         return qc_ms_dispatch<P, UPDATE_APP, MAXT, G + 1>(groups, tab, a, st, s0, dc, r, p, Z, prm);
     }
 }
@@ -430,7 +444,7 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
                                                 const Params& prm) {
     const int s0 = ld_uniform(c.qc_row_ptr, br);
     const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
-    const int groups = ((dc + P - 1) / P + 3) >> 2;
+    const int groups = ((dc + P - 1) / P + kSlotStep<P> - 1) / kSlotStep<P>;
     return qc_ms_dispatch<P, UPDATE_APP, MAXT>(groups, tab, a, st, s0, dc, r, p, c.lifting, prm);
 }
 

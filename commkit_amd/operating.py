@@ -24,7 +24,8 @@ from __future__ import annotations
 import math
 from typing import Any
 
-__all__ = ["WATERFALL_DB", "waterfall_db", "calibrate", "frame_error_rate"]
+__all__ = ["WATERFALL_DB", "SHAPED_WATERFALL_DB", "waterfall_db", "calibrate",
+           "calibrate_shaped", "frame_error_rate", "shaped_frame_error_rate"]
 
 #: Measured waterfalls (dB) by (code name, constellation order), from
 #: ``python -m commkit_amd.operating`` on an MI355X.
@@ -42,6 +43,14 @@ WATERFALL_DB: dict[tuple[str, int], float] = {
     ("r5/6-z360", 16): 11.75,
     ("r5/6-z360", 64): 17.25,
     ("r5/6-z360", 256): 22.65,
+}
+
+#: Measured waterfalls (dB) of the shaping stand-in
+#: (:func:`commkit_amd.workloads.make_pas_frames`) by (code name, QAM order,
+#: Maxwell-Boltzmann nu): exact LLRs with the prior, layered sum-product of
+#: at most 30 iterations, otherwise the definition above.
+SHAPED_WATERFALL_DB: dict[tuple[str, int, float], float] = {
+    ("r5/6-z360", 256, 0.05): 13.3,
 }
 
 _GRID = 0.05
@@ -77,6 +86,50 @@ def frame_error_rate(code: Any, constellation: Any, esn0_db: float, frames: int,
     return float(wrong.float().mean())
 
 
+def shaped_frame_error_rate(code: Any, constellation: Any, esn0_db: float, frames: int,
+                            seed: int, method: Any = None) -> float:
+    """FER of ``frames`` frames of the shaping stand-in at ``esn0_db`` (GPU)."""
+    import torch
+
+    from .coding.ldpc import SumProduct, _decode
+    from .mapping.demap import llr_any
+    from .workloads import make_pas_frames
+
+    method = method or SumProduct(max_iterations=30)
+    fr = make_pas_frames(code, constellation, esn0_db, frames, seed=seed)
+    llr = llr_any(constellation, fr.rx, float(fr.noise_var[0]), "exact").reshape(frames, code.n)
+    bits, _, _, _ = _decode(code, llr[:, fr.iperm].contiguous(), method)
+    wrong = (bits != fr.info).any(dim=1)
+    torch.cuda.synchronize()
+    return float(wrong.float().mean())
+
+
+def calibrate_shaped(code_name: str, order: int, nu: float, *, verbose: bool = True) -> float:
+    """Measure the waterfall of the shaping stand-in (SumProduct, 30 iterations)."""
+    from .mapping.constellation import Constellation
+    from .workloads import seeded_code
+
+    code = seeded_code(code_name)
+    c = Constellation.qam(order).shaped(nu=nu)
+    bits = code.rate * math.log2(order)
+    es = round(10 * math.log10(2**bits - 1) / _GRID) * _GRID
+
+    def fer(e: float) -> float:
+        idx = int(round(e / _GRID))
+        f = shaped_frame_error_rate(code, c, e, _FRAMES, seed=3000 + idx % 100000)
+        if verbose:
+            print(f"  {code_name} {order}-QAM nu={nu} Es/N0 {e:6.2f} dB: FER {f:.4f}", flush=True)
+        return f
+
+    while fer(es) > _TARGET_FER:
+        es += 0.25
+    while fer(es - 0.5) <= _TARGET_FER:  # coarse descent (shaping moves it far)
+        es -= 0.5
+    while es - _GRID > -50 and fer(es - _GRID) <= _TARGET_FER:
+        es -= _GRID
+    return round(es, 2)
+
+
 def calibrate(code_name: str, order: int, *, start_db: float | None = None,
               verbose: bool = True) -> float:
     """Measure the waterfall of a seeded code with Gray ``order``-QAM."""
@@ -110,6 +163,10 @@ if __name__ == "__main__":
 
     pairs = [("r4/5-z360", 64), ("r1/2-z27", 16)]
     if len(sys.argv) > 1:
-        pairs = [(a.split(":")[0], int(a.split(":")[1])) for a in sys.argv[1:]]
-    for name, order in pairs:
-        print(f"waterfall {name} {order}-QAM: {calibrate(name, order)} dB", flush=True)
+        pairs = [tuple(a.split(":")) for a in sys.argv[1:]]  # type: ignore[misc]
+    for p in pairs:
+        if len(p) == 3:  # name:order:nu, the shaping stand-in
+            wf = calibrate_shaped(p[0], int(p[1]), float(p[2]))
+            print(f"shaped waterfall {p[0]} {p[1]}-QAM nu={p[2]}: {wf} dB", flush=True)
+        else:
+            print(f"waterfall {p[0]} {p[1]}-QAM: {calibrate(p[0], int(p[1]))} dB", flush=True)

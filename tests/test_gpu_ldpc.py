@@ -202,3 +202,25 @@ def test_decode_contract(gpu):
 
     with pytest.warns(PrecisionWarning):
         decode(llrs.astype(np.float64).reshape(-1), code=code)
+
+
+def test_shaped_pas_frames_decode(gpu):
+    """The shaping stand-in end to end at a high Es/N0: exact LLRs with the
+    prior, gathered into codeword order, decode by layered sum-product to
+    the information bits; the amplitudes follow the prior."""
+    import torch
+
+    from commkit_amd import Constellation, SumProduct
+    from commkit_amd.mapping.demap import llr_any
+    from commkit_amd.workloads import make_pas_frames, seeded_code
+
+    code = seeded_code("r5/6-z360")
+    c = Constellation.qam(256).shaped(nu=0.05)
+    fr = make_pas_frames(code, c, 16.0, 6, seed=3)
+    assert abs(float((fr.tx.abs() ** 2).mean()) - c.power()) < 0.02
+    llr = llr_any(c, fr.rx, float(fr.noise_var[0]), "exact").reshape(6, code.n)
+    llr_c = llr[:, fr.iperm].contiguous()
+    bits, app, valid, iters = gpu_decode(code, llr_c.cpu().numpy(), SumProduct(max_iterations=30))
+    assert valid.all()
+    np.testing.assert_array_equal(bits, fr.info.cpu().numpy())
+    assert torch.equal(fr.code_bits[:, : code.k], fr.info)  # systematic
