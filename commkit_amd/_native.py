@@ -43,6 +43,15 @@ SYMBOLS: dict[str, tuple[Any, list[Any]]] = {
     "ck_batch_graph_create": (_i32, [_p, _p]),
     "ck_batch_graph_launch": (_i32, [_p, _p]),
     "ck_batch_graph_destroy": (_i32, [_p]),
+    "ck_serve_workspace_bytes": (_i64, [_p]),
+    "ck_serve_create": (_i32, [_p, _p, _p, _p]),
+    "ck_serve_admit": (_i32, [_p, _i32, _i64, ctypes.c_uint64, _p]),
+    "ck_serve_submit": (_i32, [_p, ctypes.c_uint64]),
+    "ck_serve_step": (_i32, [_p, _p]),
+    "ck_serve_poll": (_i32, [_p, _p, _i32, _p]),
+    "ck_serve_release": (_i32, [_p, ctypes.c_uint64]),
+    "ck_serve_usage": (_i32, [_p, _i32, _p, _p, _p]),
+    "ck_serve_destroy": (_i32, [_p]),
 }
 
 _LIB: ctypes.CDLL | None = None

@@ -26,6 +26,7 @@
 // counters are cleared by a small kernel launched before it (no memset
 // node).  Every workgroup exits once every queue is empty and its slots are
 // idle: no workgroup waits on another.
+#include "ck_batch_int.h"
 #include "ck_common.h"
 #include "ck_ldpc_dev.h"
 
@@ -37,7 +38,7 @@ namespace {
 
 using namespace ckdev;
 
-constexpr int kMaxClasses = 8;
+using ckbatch::kMaxClasses;
 constexpr int kBatchThreads = 768;
 
 struct BatchClass {
@@ -257,33 +258,36 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
         L.slot_bytes = std::max(L.slot_bytes, static_cast<size_t>(F) * per_slot);
     }
     L.slot_bytes = (L.slot_bytes + 255) / 256 * 256;
-    const int per_cu = std::max<int>(1, std::min<int>(2048 / kBatchThreads,
-                                                      static_cast<int>(lds_max / std::max<size_t>(L.lds_bytes, 1))));
-    L.grid = info.num_cus * per_cu;
+    // One workgroup of twelve waves per CU: its registers (up to 168 VGPRs
+    // a lane) leave no room for a second.
+    L.grid = info.num_cus;
     *out = L;
     return CK_OK;
 }
 
 }  // namespace
 
-extern "C" int64_t ck_batch_workspace_bytes(const ck_batch_class* classes, int32_t num_classes,
-                                            const ck_ldpc_method* method) {
-    Layout L{};
-    if (make_layout(classes, num_classes, method, &L) != CK_OK) return -1;
-    return static_cast<int64_t>(sizeof(BatchDesc) + L.slot_bytes * static_cast<size_t>(L.grid));
-}
+namespace ckbatch {
 
-extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_classes,
-                                    const ck_ldpc_method* method, void* workspace,
-                                    int64_t workspace_bytes, ck_batch_plan* plan) {
+size_t desc_bytes() { return sizeof(BatchDesc); }
+
+int32_t shape(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_method* method,
+              Shape* out) {
     Layout L{};
     if (int32_t st = make_layout(classes, num_classes, method, &L)) return st;
-    CK_REQUIRE(plan != nullptr, "plan is NULL");
-    const size_t need = sizeof(BatchDesc) + L.slot_bytes * static_cast<size_t>(L.grid);
-    CK_REQUIRE(workspace != nullptr && workspace_bytes >= static_cast<int64_t>(need),
-               "workspace of %lld bytes, %zu needed", static_cast<long long>(workspace_bytes), need);
-    CK_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0, "workspace not 256-byte aligned");
-    BatchDesc d;
+    out->grid = L.grid;
+    out->lds_bytes = L.lds_bytes;
+    out->slot_bytes = L.slot_bytes;
+    for (int i = 0; i < kMaxClasses; ++i) {
+        out->F[i] = L.F[i];
+        out->P[i] = L.P[i];
+    }
+    return CK_OK;
+}
+
+void describe(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_method* method,
+              const Shape& L, void* workspace, void* host_desc) {
+    BatchDesc& d = *static_cast<BatchDesc*>(host_desc);
     std::memset(&d, 0, sizeof(d));
     d.num_classes = num_classes;
     d.sum_product = method->rule == 1;
@@ -304,15 +308,61 @@ extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_c
         o.F = L.F[i];
         o.P = L.P[i];
     }
-    CK_HIP(hipMemcpy(workspace, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy(batch descriptor)");
-    if (L.lds_bytes > 65536) {
-        // Set here, outside any stream capture: the launch itself only enqueues.
-        const void* k = d.sum_product ? reinterpret_cast<const void*>(ldpc_batch<true>)
-                                      : reinterpret_cast<const void*>(ldpc_batch<false>);
+}
+
+void allow_lds(bool sum_product, size_t lds_bytes) {
+    if (lds_bytes > 65536) {
+        const void* k = sum_product ? reinterpret_cast<const void*>(ldpc_batch<true>)
+                                    : reinterpret_cast<const void*>(ldpc_batch<false>);
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(L.lds_bytes));
+                                  static_cast<int>(lds_bytes));
         (void)hipGetLastError();
     }
+}
+
+int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, hipStream_t s) {
+    BatchDesc* d = static_cast<BatchDesc*>(workspace);
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, d);
+    if (int32_t st = ck::launched("batch reset")) return st;
+    if (sum_product)
+        hipLaunchKernelGGL(ldpc_batch<true>, dim3(grid), dim3(kBatchThreads), lds_bytes, s, d);
+    else
+        hipLaunchKernelGGL(ldpc_batch<false>, dim3(grid), dim3(kBatchThreads), lds_bytes, s, d);
+    return ck::launched("batch decode");
+}
+
+}  // namespace ckbatch
+
+extern "C" int64_t ck_batch_workspace_bytes(const ck_batch_class* classes, int32_t num_classes,
+                                            const ck_ldpc_method* method) {
+    Layout L{};
+    if (make_layout(classes, num_classes, method, &L) != CK_OK) return -1;
+    return static_cast<int64_t>(sizeof(BatchDesc) + L.slot_bytes * static_cast<size_t>(L.grid));
+}
+
+extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_classes,
+                                    const ck_ldpc_method* method, void* workspace,
+                                    int64_t workspace_bytes, ck_batch_plan* plan) {
+    Layout L{};
+    if (int32_t st = make_layout(classes, num_classes, method, &L)) return st;
+    CK_REQUIRE(plan != nullptr, "plan is NULL");
+    const size_t need = sizeof(BatchDesc) + L.slot_bytes * static_cast<size_t>(L.grid);
+    CK_REQUIRE(workspace != nullptr && workspace_bytes >= static_cast<int64_t>(need),
+               "workspace of %lld bytes, %zu needed", static_cast<long long>(workspace_bytes), need);
+    CK_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0, "workspace not 256-byte aligned");
+    ckbatch::Shape shp{};
+    shp.grid = L.grid;
+    shp.lds_bytes = L.lds_bytes;
+    shp.slot_bytes = L.slot_bytes;
+    for (int i = 0; i < kMaxClasses; ++i) {
+        shp.F[i] = L.F[i];
+        shp.P[i] = L.P[i];
+    }
+    BatchDesc d;
+    ckbatch::describe(classes, num_classes, method, shp, workspace, &d);
+    CK_HIP(hipMemcpy(workspace, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy(batch descriptor)");
+    // Set here, outside any stream capture: the launch itself only enqueues.
+    ckbatch::allow_lds(d.sum_product != 0, L.lds_bytes);
     plan->workspace = workspace;
     plan->grid = L.grid;
     plan->threads = kBatchThreads;
@@ -325,16 +375,8 @@ extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_c
 extern "C" int32_t ck_batch_launch(const ck_batch_plan* plan, void* stream) {
     CK_REQUIRE(plan != nullptr && plan->workspace != nullptr, "plan is not prepared");
     CK_REQUIRE(plan->grid >= 1 && plan->threads == kBatchThreads, "plan is not prepared");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    BatchDesc* d = static_cast<BatchDesc*>(plan->workspace);
-    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, d);
-    if (int32_t st = ck::launched("batch reset")) return st;
-    const size_t lds = static_cast<size_t>(plan->lds_bytes);
-    if (plan->sum_product)
-        hipLaunchKernelGGL(ldpc_batch<true>, dim3(plan->grid), dim3(kBatchThreads), lds, s, d);
-    else
-        hipLaunchKernelGGL(ldpc_batch<false>, dim3(plan->grid), dim3(kBatchThreads), lds, s, d);
-    return ck::launched("batch decode");
+    return ckbatch::launch(plan->workspace, plan->grid, static_cast<size_t>(plan->lds_bytes),
+                           plan->sum_product != 0, static_cast<hipStream_t>(stream));
 }
 
 namespace {

@@ -29,7 +29,8 @@ enum ck_status {
     CK_OK = 0,
     CK_EINVAL = 1,   /* bad argument (shape, size, null pointer, range) */
     CK_EHIP = 2,     /* HIP runtime error (launch, attribute) */
-    CK_ENOSPACE = 3  /* problem exceeds a fixed limit of the library */
+    CK_ENOSPACE = 3, /* problem exceeds a fixed limit of the library */
+    CK_EBUSY = 4     /* no room now (serving: the request's ring is full); retry later */
 };
 
 /* Library version (major * 10000 + minor * 100 + patch). */
@@ -240,6 +241,85 @@ int32_t ck_batch_launch(const ck_batch_plan* plan, void* stream);
 int32_t ck_batch_graph_create(const ck_batch_plan* plan, void** graph);
 int32_t ck_batch_graph_launch(void* graph, void* stream);
 int32_t ck_batch_graph_destroy(void* graph);
+
+/* ---------------------------------------------------------------------
+ * Serving engine: continuous batching of decode requests over the
+ * batched decoder, admission against a device-memory budget.  No
+ * counterpart in the reference (it decodes one call at a time,
+ * commkit/coding/base.py:317-379); requests are what a caller of
+ * `decode` would pass, frames of one code each.
+ *
+ * The caller owns all memory: per code a ring of `capacity` frame
+ * buffers, sized from its budget, and `max_inflight` launch workspaces of
+ * ck_serve_workspace_bytes() bytes (256-byte aligned).  One host thread
+ * drives an engine.
+ * ------------------------------------------------------------------- */
+
+typedef struct ck_serve ck_serve;
+
+typedef struct ck_serve_ring {  /* device buffers of `capacity` frames   */
+    float* llrs;                /* (capacity, n) channel LLRs (written by the caller) */
+    float* app;                 /* (capacity, n) APP LLRs                */
+    int8_t* bits;               /* (capacity, k) hard info bits          */
+    int32_t* iterations;        /* (capacity,)                           */
+    uint8_t* valid;             /* (capacity,)                           */
+    int64_t capacity;
+} ck_serve_ring;
+
+typedef struct ck_serve_config {
+    const ck_ldpc_code* const* codes;  /* num_codes codes                 */
+    const ck_serve_ring* rings;        /* one ring per code               */
+    int32_t num_codes;                 /* 1..8                            */
+    int32_t max_inflight;              /* launches in flight, 1..8        */
+    const ck_ldpc_method* method;      /* layered                         */
+    int32_t dry_run;                   /* 1: no device work (host logic only;
+                                          buffers may be NULL)            */
+} ck_serve_config;
+
+typedef struct ck_serve_slot {  /* where an admitted request's frames live */
+    float* llrs;
+    float* app;
+    int8_t* bits;
+    int32_t* iterations;
+    uint8_t* valid;
+    int64_t first;              /* first frame's index in the ring       */
+} ck_serve_slot;
+
+/* Bytes of one launch workspace for this configuration (the rings'
+ * buffers may still be NULL); -1 on error. */
+int64_t ck_serve_workspace_bytes(const ck_serve_config* config);
+
+/* workspaces: max_inflight device pointers (ignored when dry_run). */
+int32_t ck_serve_create(const ck_serve_config* config, void* const* workspaces,
+                        void* stream, ck_serve** out);
+
+/* Reserves `frames` contiguous ring slots of `code` for request `tag`
+ * (unique among live requests); CK_EBUSY when they do not fit now. */
+int32_t ck_serve_admit(ck_serve* server, int32_t code, int64_t frames, uint64_t tag,
+                       ck_serve_slot* where);
+
+/* The request's LLRs are in place (written on, or ordered before work on,
+ * the engine's stream). */
+int32_t ck_serve_submit(ck_serve* server, uint64_t tag);
+
+/* Launches every submitted request not yet launched (up to 8 contiguous
+ * runs) if a workspace is free; *launched_frames = frames launched (0:
+ * nothing to launch, or every workspace in flight). */
+int32_t ck_serve_step(ck_serve* server, int64_t* launched_frames);
+
+/* Up to max_tags requests completed since the last poll (non-blocking). */
+int32_t ck_serve_poll(ck_serve* server, uint64_t* tags, int32_t max_tags, int32_t* count);
+
+/* The caller is done with a completed request's outputs; its slots return
+ * to the ring (in admission order). */
+int32_t ck_serve_release(ck_serve* server, uint64_t tag);
+
+/* Ring slots in use and capacity of `code`, launches in flight. */
+int32_t ck_serve_usage(const ck_serve* server, int32_t code, int64_t* in_use,
+                       int64_t* capacity, int32_t* inflight);
+
+/* Waits for launches in flight, frees the engine's host resources. */
+int32_t ck_serve_destroy(ck_serve* server);
 
 #ifdef __cplusplus
 }
