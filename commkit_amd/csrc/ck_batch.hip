@@ -143,7 +143,7 @@ __device__ __attribute__((noinline)) void run_class(BatchDesc* __restrict__ d, i
             if (active) {
                 const bool first = my_it == 0;
                 if (SP) {
-                    qc_sp_row<true>(c, s_tab, a, Rm, br, l.r, first);
+                    qc_sp_row<1, true>(c, s_tab, a, Rm, br, l.r, 0, first);
                 } else {
                     const CheckState st = first ? zero_state() : st_next;
                     if (br + 1 < mb) {

@@ -9,7 +9,9 @@
 // float32 sum of phi(|Q|) with phi(x) = -log tanh(x/2), x clipped to
 // [1e-7, 30].  Pass 2 writes R_new = sign * mag and APP = Q + R_new, with
 // mag = max(scale * min - offset, 0) (min2 for edge idx) or
-// phi(total - phi(|Q|)).  This translation unit is compiled with
+// phi(total - phi(|Q|)); the quasi-cyclic kernels evaluate phi from the
+// hardware exp2 / log2 (phi_fast, within the sum-product tolerance), the
+// generic kernel with logf / tanhf.  This translation unit is compiled with
 // -ffp-contract=off, so every product and sum rounds on its own, as in the
 // reference: min-sum is bit-identical to the CPU restatement, including
 // iteration counts and valid flags.  Early stop checks every row's parity of
@@ -19,9 +21,9 @@
 //   qc_layered  quasi-cyclic codes.  A workgroup decodes F frames whose
 //               APPs sit in LDS; lane (f, r, p) owns row r of the current
 //               block row for frame f, with P lanes per row splitting its
-//               edges and combining min1/min2/idx/sign with wavefront
-//               shuffles (an exact, order-independent reduction on
-//               (|Q|, edge) pairs).  Columns come from the circulant shift,
+//               edges and combining min1/min2/idx/sign with DPP quad moves
+//               (an exact, order-independent reduction on (|Q|, edge)
+//               pairs; for sum-product the phi sum and sign parity).  Columns come from the circulant shift,
 //               (r + s) mod Z, so consecutive lanes touch consecutive LDS
 //               words.  Block rows run in schedule order (a layer is a union
 //               of whole block rows, whose rows share no variable), with a
@@ -50,12 +52,13 @@ namespace {
 
 using namespace ckdev;
 
-// SHORT (min-sum, rows of at most kShortRow edges): at most 128 VGPRs for
-// P = 1 (two workgroups of up to six waves per CU), 80 for P = 2 (two of up
-// to twelve).
-template <bool SP, int P, bool STATE_LDS, bool SHORT>
+// SHORT (min-sum, rows of at most kShortRow edges): row instantiations up
+// to kShortRow / P slots, with a register budget of 168 VGPRs (SHORT = 1:
+// one workgroup of up to twelve waves per CU) or 128 (SHORT = 2, P = 1: two
+// workgroups of up to six waves).
+template <bool SP, int P, bool STATE_LDS, int SHORT>
 __global__ __launch_bounds__(kMaxQcThreads)
-__attribute__((amdgpu_waves_per_eu(SHORT ? 4 * P - (P > 1 ? 2 : 0) : 1))) void ldpc_qc_layered(
+__attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void ldpc_qc_layered(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm) {
@@ -110,7 +113,7 @@ __attribute__((amdgpu_waves_per_eu(SHORT ? 4 * P - (P > 1 ? 2 : 0) : 1))) void l
         for (int br = 0; br < mb; ++br) {
             if (!done) {
                 if (SP) {
-                    qc_sp_row<true>(c, s_tab, a, Rm, br, l.r, first);
+                    qc_sp_row<P, true>(c, s_tab, a, Rm, br, l.r, l.p, first);
                 } else {
                     const CheckState st = st_next;
                     if (br + 1 < mb) {
@@ -120,7 +123,7 @@ __attribute__((amdgpu_waves_per_eu(SHORT ? 4 * P - (P > 1 ? 2 : 0) : 1))) void l
                         st_next = st_base[l.r];  // block row 0, stored this iteration
                     }
                     const CheckState ns =
-                        qc_ms_row<P, true, SHORT ? kShortRow / P : kMaxDegree / P>(c, s_tab, a, st, br, l.r, l.p, prm);
+                        qc_ms_row<P, true, SHORT != 0 ? kShortRow / P : kMaxDegree / P>(c, s_tab, a, st, br, l.r, l.p, prm);
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
                 }
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         if (!done) {
             for (int br = 0; br < c.base_rows; ++br) {
                 if (SP) {
-                    if (l.p == 0) qc_sp_row<false>(c, s_tab, a, Rm, br, l.r, first);
+                    if (l.p == 0) qc_sp_row<1, false>(c, s_tab, a, Rm, br, l.r, 0, first);
                 } else {
                     CheckState* sp = st_base + static_cast<size_t>(br) * Z + l.r;
                     const CheckState st = first ? zero_state() : *sp;
@@ -343,7 +346,8 @@ struct Plan {
     int F;             // QC frames per workgroup
     int P;             // QC lanes per row
     bool state_lds;    // QC layered min-sum
-    bool short_rows;   // QC layered min-sum, P = 1, rows of at most kShortRow edges
+    int short_rows;    // QC layered min-sum, rows of at most kShortRow edges: the
+                       // SHORT variant (1: 168 VGPRs, 2: 128 VGPRs), else 0
     int threads;
     size_t lds_bytes;
     size_t ws_per_frame;  // bytes
@@ -437,7 +441,7 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, P
         p.kind = 2;
         int P = m->lanes_per_row;
         if (P == 0) P = 1;
-        if (p.sp) P = 1;
+        if (p.sp && p.flood) P = 1;
         if (Z * P > kMaxQcThreads) P = 1;
         int F = m->frames_per_group;
         if (F == 0) {
@@ -466,9 +470,15 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, P
         p.F = F;
         p.P = P;
         p.state_lds = state_lds;
-        p.short_rows = !p.sp && !p.flood && P <= 2 && c->max_row_degree <= kShortRow;
         p.threads = static_cast<int>(round_up(static_cast<size_t>(F) * Z * P, 64));
         p.lds_bytes = qc_lds(F, N, Z, c->num_checks, state_lds).end * 4;
+        p.short_rows = 0;
+        if (!p.sp && !p.flood && P <= 2 && c->max_row_degree <= kShortRow) {
+            // Two workgroups per CU when at most six waves each and LDS allows.
+            const bool two = P == 1 && p.threads <= 384 &&
+                             2 * (p.lds_bytes + kStaticLds) <= info.lds_per_block;
+            p.short_rows = two ? 2 : 1;
+        }
         if (p.sp)
             p.ws_per_frame = static_cast<size_t>(c->num_edges) * 4;
         else
@@ -496,7 +506,7 @@ void allow_lds(K kernel, size_t bytes) {
     }
 }
 
-template <bool SP, int P, bool SL, bool SHORT = false>
+template <bool SP, int P, bool SL, int SHORT = 0>
 void launch_qc_layered(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
                        signed char* bits, int* iters, unsigned char* valid, void* ws, long long R,
                        const Params& prm, hipStream_t s) {
@@ -569,16 +579,21 @@ extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method
             else if (p.P == 2) launch_qc_flooding<false, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_flooding<false, 4>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else if (p.sp) {
-            launch_qc_layered<true, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.P == 1) launch_qc_layered<true, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.P == 2) launch_qc_layered<true, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else launch_qc_layered<true, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else if (p.state_lds) {
-            if (p.short_rows && p.P == 1) launch_qc_layered<false, 1, true, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows == 2 && p.P == 1) launch_qc_layered<false, 1, true, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.short_rows == 1 && p.P == 1) launch_qc_layered<false, 1, true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 1) launch_qc_layered<false, 1, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 2) launch_qc_layered<false, 2, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_layered<false, 4, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else {
-            if (p.short_rows && p.P == 1) launch_qc_layered<false, 1, false, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows == 2 && p.P == 1) launch_qc_layered<false, 1, false, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.short_rows == 1 && p.P == 1) launch_qc_layered<false, 1, false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 1) launch_qc_layered<false, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, false, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else if (p.P == 2) launch_qc_layered<false, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_layered<false, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         }

@@ -448,48 +448,99 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
     return qc_ms_dispatch<P, UPDATE_APP, MAXT>(groups, tab, a, st, s0, dc, r, p, c.lifting, prm);
 }
 
-// One sum-product check row; Rm is the frame's messages, [slot][Z].  The old
-// messages of the row are loaded together up front; Q, phi(|Q|) and the LDS
-// address of every edge stay in registers between the passes.
-template <bool UPDATE_APP>
-__device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab, float* a,
-                                          float* __restrict__ Rm, int br, int r, bool first) {
-    constexpr int MAXT = kMaxDegree;
-    const int Z = c.lifting;
-    const int s0 = ld_uniform(c.qc_row_ptr, br);
-    const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
-    float rold[MAXT];
-#pragma unroll
-    for (int t = 0; t < MAXT; ++t)
-        if (t < dc) rold[t] = first ? 0.0f : Rm[static_cast<size_t>(s0 + t) * Z + r];
-    float q[MAXT];
-    float ph[MAXT];
-    int addr[MAXT];
-    float total = 0.0f;
-    unsigned neg = 0u;
+// phi(x) = -ln(tanh(x / 2)) = ln((1 + e^-x) / (1 - e^-x)) for x clipped to
+// [kPhiMin, kPhiMax], from the hardware exp2, reciprocal and log2.  Below
+// 1/16, 1 - e^-x would lose digits to cancellation and comes from its Taylor
+// series (relative error of the four terms below 2e-7 there).  Within about
+// 1e-6 relative of the float64 value where phi is above 1e-2; smaller values
+// (x above ~5) carry an absolute error near 1e-7.
+__device__ __forceinline__ float phi_fast(float x) {
+    x = fminf(fmaxf(x, kPhiMin), kPhiMax);
+    const float t = __builtin_amdgcn_exp2f(x * -1.44269504f);  // e^-x
+    const float series = x * (1.0f - x * (0.5f - x * (0.16666667f - x * 0.041666668f)));
+    const float den = x < 0.0625f ? series : 1.0f - t;
+    return __builtin_amdgcn_logf((1.0f + t) * __builtin_amdgcn_rcpf(den)) * 0.69314718f;
+}
+
+// One sum-product check row over NT slots of the lane (edges e = t*P + p);
+// Rm is the frame's messages, [slot][Z].  Branch-free like qc_ms_row_n: the
+// old messages of the lane's edges are loaded together up front (slots past
+// the row's end load a real slot of the row and are ignored); Q, phi(|Q|)
+// and the LDS address of every edge stay in registers between the passes.
+// With P lanes a row's phi sum and sign parity are combined across the
+// lanes (the sum in a different order than one lane's, within the
+// tolerance of sum-product).
+template <int P, int NT, bool UPDATE_APP>
+__device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __restrict__ Rm,
+                                            int s0, int dc, int r, int p, int Z, bool first) {
+    constexpr int S = kSlotStep<P>;
+    float q[NT];
+    float ph[NT];
+    float* addr[NT];
     float* ar = a + r;
     const int z4 = 4 * Z;
     const unsigned rr = row_rank(r);
+    float rold[NT];
 #pragma unroll
-    for (int t = 0; t < MAXT; ++t) {
-        if (t < dc) {
-            addr[t] = qc_off(tab[s0 + t], rr, z4);
-            q[t] = lds_at(ar, addr[t]) - rold[t];
-            neg ^= (q[t] < 0.0f) ? 1u : 0u;
-            ph[t] = phi(fabsf(q[t]));
-            total = total + ph[t];
-        }
+    for (int t = 0; t < NT; ++t) {
+        const int e = t * P + p;
+        const bool valid = (t < NT - S) || (e < dc);
+        addr[t] = &lds_at(ar, qc_off(tab[s0 + t * P + p], rr, z4));
+        rold[t] = first ? 0.0f : Rm[static_cast<size_t>(valid ? s0 + e : s0) * Z + r];
+    }
+    float total = 0.0f;
+    unsigned neg = 0u;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int e = t * P + p;
+        const bool valid = (t < NT - S) || (e < dc);
+        q[t] = *addr[t] - rold[t];
+        ph[t] = valid ? phi_fast(fabsf(q[t])) : 0.0f;
+        total = total + ph[t];
+        neg ^= (valid && q[t] < 0.0f) ? 1u : 0u;
     }
 #pragma unroll
-    for (int t = 0; t < MAXT; ++t) {
-        if (t < dc) {
-            const float mag = phi(total - ph[t]);
+    for (int off = 1; off < P; off <<= 1) {
+        total = total + __int_as_float(lane_xor(__float_as_int(total), off));
+        neg ^= static_cast<unsigned>(lane_xor(static_cast<int>(neg), off));
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int e = t * P + p;
+        if ((t < NT - S) || (e < dc)) {
+            const float mag = phi_fast(total - ph[t]);
             const float rn = (neg ^ ((q[t] < 0.0f) ? 1u : 0u)) ? -mag : mag;
-            if (UPDATE_APP) lds_at(ar, addr[t]) = q[t] + rn;
-            Rm[static_cast<size_t>(s0 + t) * Z + r] = rn;
+            if (UPDATE_APP) *addr[t] = q[t] + rn;
+            Rm[static_cast<size_t>(s0 + e) * Z + r] = rn;
         }
     }
 }
 
+template <int P, bool UPDATE_APP, int MAXT, int G = 1>
+__device__ __forceinline__ void qc_sp_dispatch(int groups, const int2* tab, float* a,
+                                               float* __restrict__ Rm, int s0, int dc, int r, int p,
+                                               int Z, bool first) {
+    constexpr int S = kSlotStep<P>;
+    if constexpr (S * G >= MAXT) {
+        qc_sp_row_n<P, S * G, UPDATE_APP>(tab, a, Rm, s0, dc, r, p, Z, first);
+    } else {
+        if (groups <= G) {
+            qc_sp_row_n<P, S * G, UPDATE_APP>(tab, a, Rm, s0, dc, r, p, Z, first);
+            return;
+        }
+        qc_sp_dispatch<P, UPDATE_APP, MAXT, G + 1>(groups, tab, a, Rm, s0, dc, r, p, Z, first);
+    }
+}
+
+// One sum-product check row `r` of block row `br`.
+template <int P, bool UPDATE_APP>
+__device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab, float* a,
+                                          float* __restrict__ Rm, int br, int r, int p, bool first) {
+    const int s0 = ld_uniform(c.qc_row_ptr, br);
+    const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
+    const int groups = ((dc + P - 1) / P + kSlotStep<P> - 1) / kSlotStep<P>;
+    qc_sp_dispatch<P, UPDATE_APP, kMaxDegree / P>(groups, tab, a, Rm, s0, dc, r, p, c.lifting,
+                                                  first);
+}
 
 }  // namespace ckdev
