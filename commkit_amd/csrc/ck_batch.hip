@@ -231,12 +231,7 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
         // APP plus sign bitmaps of one frame.
         const size_t app_bytes = qc_lds(1, c.num_variables, c.lifting, c.num_checks, false).end * 4;
         int P = bc.lanes_per_row;
-        if (P == 0) {
-            // Two lanes per row when LDS, not lanes, limits the slots.
-            const int f1 = std::min<int>(kBatchThreads / c.lifting,
-                                         static_cast<int>(lds_max / app_bytes));
-            P = (f1 * c.lifting * 2 <= kBatchThreads) ? 2 : 1;
-        }
+        if (P == 0) P = (2 * c.lifting <= kBatchThreads) ? 2 : 1;  // twice the lanes per frame
         if (sp) P = 1;
         CK_REQUIRE(P == 1 || P == 2, "class %d: lanes_per_row %d not 1 or 2", i, P);
         CK_REQUIRE(c.lifting * P <= kBatchThreads, "class %d: lifting %d x %d lanes above %d", i,

@@ -333,9 +333,9 @@ __device__ __forceinline__ CheckState zero_state() {
 }
 
 // Slot granularity of the row instantiations: rows are padded to a multiple
-// of kSlotStep slots per lane (four for one lane per row, two for several).
+// of kSlotStep slots per lane.
 template <int P>
-constexpr int kSlotStep = P == 1 ? 4 : 2;
+constexpr int kSlotStep = 2;
 
 // One min-sum check row of a block row over NT slots of the lane (edges
 // e = t*P + p), given its old state; returns the new state.  Branch-free:
