@@ -209,7 +209,8 @@ __device__ __forceinline__ float with_sign(float mag, unsigned bits, int e) {
 // a slot) says whether the slot is checked.  Called by every lane (barriers);
 // lanes outside the F slots (lane_ok false) return true.
 //
-// The waves first write the bitmaps, one ballot of 64 signs per step; then
+// A sample of rows is checked first (quick rejection).  Otherwise the waves
+// write the bitmaps, one ballot of 64 signs per step; then
 // each task (slot, block row, 32 rows from r0) XORs, over the block row's
 // circulants {j, shift}, the 32 bits from column (r0 + shift) mod Z of
 // bitmap j: bit i of the result is the parity of row r0 + i.
@@ -227,6 +228,32 @@ __device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const flo
     const int per_frame = c.base_rows * K;
     const int checks = F * per_frame;
     if (lane_ok && l.u == 0) s_flag[l.f] = test ? 0 : 2;
+    __syncthreads();
+    // Quick rejection: lanes (f, r < 64, p = 0) check row r of block row 0
+    // straight from the APP; one unsatisfied row settles the frame, which
+    // then skips the full check.  While a frame is far from converged this
+    // almost always happens, and the bitmaps are not built at all.
+    if (lane_ok && l.p == 0 && l.r < 64 && s_flag[l.f] == 0) {
+        const int s0 = ld_uniform(c.qc_row_ptr, 0);
+        const int s1 = ld_uniform(c.qc_row_ptr, 1);
+        float* ar = const_cast<float*>(app) + static_cast<size_t>(l.f) * N + l.r;
+        const unsigned rr = row_rank(l.r);
+        const int z4 = 4 * Z;
+        unsigned parity = 0u;
+        for (int s = s0; s < s1; s += U) {
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = lds_at(ar, qc_off(tab[s + u], rr, z4));  // padded table
+#pragma unroll
+            for (int u = 0; u < U; ++u) parity ^= (s + u < s1 && v[u] < 0.0f) ? 1u : 0u;
+        }
+        if (parity) s_flag[l.f] = 1;
+    }
+    if (!__syncthreads_or(lane_ok && l.u == 0 && s_flag[l.f] == 0)) {
+        const bool ok = lane_ok ? (s_flag[l.f] == 0) : true;
+        __syncthreads();
+        return ok;
+    }
     // The first check task of this lane; its row range is fetched now so the
     // load completes under the bitmap writes.
     int task = threadIdx.x;
