@@ -21,8 +21,6 @@ from __future__ import annotations
 import ctypes
 from typing import Any, Sequence
 
-import numpy as np
-
 from . import _native
 from .coding.ldpc import LDPCCode, MinSum, SumProduct, device_code, method_struct
 
@@ -173,8 +171,3 @@ class BatchDecoder:
             self.close()
         except Exception:
             pass
-
-
-def class_bits_valid(res: BatchResult) -> tuple[np.ndarray, np.ndarray]:
-    """Host copies of a class's bits and valid flags."""
-    return res.bits.cpu().numpy(), res.valid.cpu().numpy().astype(bool)

@@ -5,7 +5,8 @@
 // contiguous).  The launch is a persistent grid, one workgroup per CU (or as
 // many as LDS allows).  A workgroup works through the classes in order; for
 // class c it holds F_c frame slots (the APP of each in LDS, its check state
-// in a per-slot workspace) and decodes them with the layered row updates of
+// in LDS too when it fits, else in a per-slot workspace) and decodes them
+// with the layered row updates of
 // the per-call QC kernel, lane (f, r, p) owning row r of slot f.  Each slot
 // takes its next frame from the class's queue (one atomic counter per class
 // in the workspace): at every iteration boundary a slot whose frame has
@@ -51,6 +52,7 @@ struct BatchClass {
     long long num_frames;
     int F;
     int P;
+    int state_lds;  // min-sum check states in LDS (else in the slot workspace)
 };
 
 struct alignas(256) BatchDesc {
@@ -109,7 +111,7 @@ __device__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_co
     }
 }
 
-template <bool SP, int P>
+template <bool SP, int P, bool STATE_LDS>
 __device__ __attribute__((noinline)) void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s_tab, int* s_need,
                           int* s_frame, int* s_flag) {
     const BatchClass bc = d->cls[ci];
@@ -126,7 +128,10 @@ __device__ __attribute__((noinline)) void run_class(BatchDesc* __restrict__ d, i
     float* a = lds + static_cast<size_t>(slot) * N;
     unsigned* bits = reinterpret_cast<unsigned*>(lds + qc_lds(F, N, Z, c.num_checks, false).bits);
     char* ws = d->slots + static_cast<size_t>(blockIdx.x) * d->slot_bytes;
-    CheckState* st_base = reinterpret_cast<CheckState*>(ws) + static_cast<size_t>(slot) * c.num_checks;
+    CheckState* st_base =
+        (STATE_LDS ? reinterpret_cast<CheckState*>(lds + qc_lds(F, N, Z, c.num_checks, true).state)
+                   : reinterpret_cast<CheckState*>(ws)) +
+        static_cast<size_t>(slot) * c.num_checks;
     float* Rm = reinterpret_cast<float*>(ws) + static_cast<size_t>(slot) * c.num_edges;
 
     __syncthreads();  // the previous class is done with LDS
@@ -182,9 +187,15 @@ __global__ __launch_bounds__(kBatchThreads) void ldpc_batch(BatchDesc* __restric
     const int nc = d->num_classes;
     for (int ci = 0; ci < nc; ++ci) {
         if (d->cls[ci].P == 2)
-            run_class<SP, 2>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+            if (!SP && d->cls[ci].state_lds)
+                run_class<SP, 2, true>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+            else
+                run_class<SP, 2, false>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
         else
-            run_class<SP, 1>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+            if (!SP && d->cls[ci].state_lds)
+                run_class<SP, 1, true>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+            else
+                run_class<SP, 1, false>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
     }
 }
 
@@ -194,6 +205,7 @@ struct Layout {
     size_t slot_bytes;
     int F[kMaxClasses];
     int P[kMaxClasses];
+    int S[kMaxClasses];  // check states in LDS
 };
 
 int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_method* m,
@@ -244,12 +256,16 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
                    i, app_bytes);
         while (F > 1 && qc_lds(F, c.num_variables, c.lifting, c.num_checks, false).end * 4 > lds_max)
             --F;
+        // Min-sum check states in LDS when they fit beside the slots' APPs.
+        const bool st_lds =
+            !sp && qc_lds(F, c.num_variables, c.lifting, c.num_checks, true).end * 4 <= lds_max;
         L.F[i] = F;
         L.P[i] = P;
+        L.S[i] = st_lds ? 1 : 0;
         L.lds_bytes = std::max(L.lds_bytes,
-                               qc_lds(F, c.num_variables, c.lifting, c.num_checks, false).end * 4);
+                               qc_lds(F, c.num_variables, c.lifting, c.num_checks, st_lds).end * 4);
         const size_t per_slot = sp ? static_cast<size_t>(c.num_edges) * 4
-                                   : static_cast<size_t>(c.num_checks) * sizeof(CheckState);
+                                   : (st_lds ? 0 : static_cast<size_t>(c.num_checks) * sizeof(CheckState));
         L.slot_bytes = std::max(L.slot_bytes, static_cast<size_t>(F) * per_slot);
     }
     L.slot_bytes = (L.slot_bytes + 255) / 256 * 256;
@@ -276,6 +292,7 @@ int32_t shape(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
     for (int i = 0; i < kMaxClasses; ++i) {
         out->F[i] = L.F[i];
         out->P[i] = L.P[i];
+        out->S[i] = L.S[i];
     }
     return CK_OK;
 }
@@ -302,6 +319,7 @@ void describe(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
         o.num_frames = bc.num_frames;
         o.F = L.F[i];
         o.P = L.P[i];
+        o.state_lds = L.S[i];
     }
 }
 
@@ -352,6 +370,7 @@ extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_c
     for (int i = 0; i < kMaxClasses; ++i) {
         shp.F[i] = L.F[i];
         shp.P[i] = L.P[i];
+        shp.S[i] = L.S[i];
     }
     BatchDesc d;
     ckbatch::describe(classes, num_classes, method, shp, workspace, &d);

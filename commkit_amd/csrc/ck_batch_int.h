@@ -19,6 +19,7 @@ struct Shape {
     size_t slot_bytes;     // check-state workspace of a workgroup
     int F[kMaxClasses];    // frame slots per workgroup, per class
     int P[kMaxClasses];    // lanes per check row, per class
+    int S[kMaxClasses];    // 1: the class's min-sum check states live in LDS
 };
 
 // Bytes of the descriptor at the start of a launch's workspace; the

@@ -30,7 +30,10 @@ def _llrs(code, rows, seed):
 
 
 def _alone(code, x, method):
-    bits, app, valid, iters = _decode(code, x, method)
+    # The batched decoder runs sum-product with one lane per row; the per-call
+    # reference must add a row's phi terms in the same order (P = 1).
+    tiles = (2, 0, 1, 0) if isinstance(method, SumProduct) else None
+    bits, app, valid, iters = _decode(code, x, method, tiles=tiles)
     return [t.cpu().numpy() for t in (bits, app, valid.to(dtype=bits.dtype), iters)]
 
 
