@@ -55,8 +55,14 @@ def _parse() -> argparse.Namespace:
                     help="target CPU time of the baseline sample (5 timed runs)")
     ap.add_argument("--loads", type=str, default="0.3,0.6,0.9",
                     help="serving: offered loads as fractions of the saturated throughput")
-    ap.add_argument("--serve-seconds", type=float, default=2.0,
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="serving: seconds of arrivals per offered load")
+    ap.add_argument("--max-inflight", type=int, default=2,
+                    help="serving: launches in flight (one workspace and stream each)")
+    ap.add_argument("--bound-load", type=float, default=0.6,
+                    help="serving: offered load (fraction of saturated) the p99 bound is stated at")
+    ap.add_argument("--p99-bound-ms", type=float, default=10.0,
+                    help="serving: stated p99 frame-latency bound at --bound-load")
     ap.add_argument("--workload", choices=("single", "mixed", "shaped", "serving"),
                     default="single",
                     help="single: one code and constellation (default); mixed: 10,000 frames "
@@ -422,7 +428,7 @@ def run_serving(args: argparse.Namespace) -> dict:
               file=sys.stderr, flush=True)  # fmt: skip
     torch.cuda.synchronize()
     method = MinSum(max_iterations=50)
-    srv = Server(codes, method, hbm_budget=16 << 30, max_inflight=2, device=dev)
+    srv = Server(codes, method, hbm_budget=16 << 30, max_inflight=args.max_inflight, device=dev)
     mean_bits = float(np.mean([codes[ci].k for ci, _ in pairs])) * (1 + max_frames) / 2
 
     def make_fill(sched: Schedule):
@@ -446,7 +452,10 @@ def run_serving(args: argparse.Namespace) -> dict:
     print(json.dumps({"progress": "saturated", "gbps": round(sat.gbps, 3),
                       "launches": sat.launches}), file=sys.stderr, flush=True)  # fmt: skip
     loads = []
-    for frac in [float(x) for x in args.loads.split(",") if x]:
+    fracs = [float(x) for x in args.loads.split(",") if x]
+    if args.bound_load not in fracs:
+        fracs = sorted(fracs + [args.bound_load])
+    for frac in fracs:
         rate = frac * sat.gbps * 1e9 / mean_bits
         sched = codes_of(poisson_schedule(rate, args.serve_seconds, streams=streams, num_codes=3,
                                           max_frames=max_frames, seed=int(frac * 1000)))  # fmt: skip
@@ -465,6 +474,10 @@ def run_serving(args: argparse.Namespace) -> dict:
         print(json.dumps({"progress": "load", **loads[-1]}), file=sys.stderr, flush=True)
     srv.close()
     top = loads[-1] if loads else None
+    at = next(x for x in loads if x["offered_fraction"] == args.bound_load)
+    bound = {"offered_fraction": args.bound_load, "offered_gbps": at["offered_gbps"],
+             "p99_bound_ms": args.p99_bound_ms, "p99_ms": at["frame_latency_ms"]["p99"],
+             "met": at["frame_latency_ms"]["p99"] <= args.p99_bound_ms}
     return {
         "metric": "decoded info Gb/s at <=1e-5 post-FEC BER; p50/p99 frame latency",
         "value": round(sat.gbps, 3),
@@ -484,7 +497,10 @@ def run_serving(args: argparse.Namespace) -> dict:
                         "in [wf-0.5, wf+1] dB, 16 GiB HBM budget, layered NMS 0.875 <=50 "
                         "iterations, continuous batching",
             "saturation_requests": n_sat,
+            "max_inflight": args.max_inflight,
+            "serve_seconds": args.serve_seconds,
         },
+        "latency_bound": bound,
         "frame_latency_ms": top["frame_latency_ms"] if top else None,
         "saturated": {"gbps": round(sat.gbps, 3), "requests": sat.requests,
                       "launches": sat.launches,

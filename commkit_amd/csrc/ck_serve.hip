@@ -20,8 +20,11 @@
 //
 // Requests therefore join the next launch as soon as they arrive instead of
 // waiting for a full batch, and a launch in flight never blocks admission
-// or submission.  Launches run in stream order on one stream; each records
-// an event that poll() queries without blocking.  A frame decodes the same
+// or submission.  Each workspace has its own stream, so launches in flight
+// overlap on the device (a new launch's workgroups take the CUs that the
+// previous one's finished workgroups free); a launch first waits for the
+// work enqueued on the caller's stream (the requests' LLRs), and records an
+// event that poll() queries without blocking.  A frame decodes the same
 // alone or in any launch (the batched decoder's contract), so a request's
 // results do not depend on the traffic it was served with.
 //
@@ -61,9 +64,10 @@ struct Ring {
 struct Launch {
     bool busy = false;
     long long seq = 0;
+    hipStream_t stream = nullptr;  // own stream: launches in flight overlap on the device
     hipEvent_t done = nullptr;
-    void* host_desc = nullptr;   // pinned: read by the descriptor copy
-    void* workspace = nullptr;   // device
+    void* host_desc = nullptr;     // pinned: read by the descriptor copy
+    void* workspace = nullptr;     // device
 };
 
 }  // namespace
@@ -73,7 +77,8 @@ struct ck_serve {
     ck_ldpc_method method{};
     std::vector<Ring> rings;
     std::vector<Launch> launches;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // the caller's stream, where request LLRs are written
+    hipEvent_t inputs = nullptr;      // recorded there at each launch, waited on by the launch
     bool dry = false;
     long long next_seq = 0;
     int64_t workspace_bytes = 0;
@@ -209,7 +214,13 @@ extern "C" int32_t ck_serve_create(const ck_serve_config* config, void* const* w
             if (st == CK_OK)
                 st = ck::hip_status(hipEventCreateWithFlags(&l.done, hipEventDisableTiming),
                                     "hipEventCreate");
+            if (st == CK_OK)  // default priority, like every stream of this library
+                st = ck::hip_status(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking),
+                                    "hipStreamCreate");
         }
+        if (st == CK_OK)
+            st = ck::hip_status(hipEventCreateWithFlags(&s->inputs, hipEventDisableTiming),
+                                "hipEventCreate");
     }
     if (st != CK_OK) {
         (void)ck_serve_destroy(s);
@@ -337,13 +348,17 @@ extern "C" int32_t ck_serve_step(ck_serve* s, int64_t* launched_frames) {
                    "launch workspace %zu above the bound %lld", need,
                    static_cast<long long>(s->workspace_bytes));
         ckbatch::describe(cls, n, &s->method, sh, l.workspace, l.host_desc);
+        // The launch follows everything enqueued so far on the caller's stream
+        // (the requests' LLRs), then runs on its own stream.
+        CK_HIP(hipEventRecord(s->inputs, s->stream), "hipEventRecord(inputs)");
+        CK_HIP(hipStreamWaitEvent(l.stream, s->inputs, 0), "hipStreamWaitEvent");
         CK_HIP(hipMemcpyAsync(l.workspace, l.host_desc, ckbatch::desc_bytes(),
-                              hipMemcpyHostToDevice, s->stream),
+                              hipMemcpyHostToDevice, l.stream),
                "hipMemcpyAsync(descriptor)");
         if (int32_t st = ckbatch::launch(l.workspace, sh.grid, sh.lds_bytes, s->method.rule == 1,
-                                         s->stream))
+                                         l.stream))
             return st;
-        CK_HIP(hipEventRecord(l.done, s->stream), "hipEventRecord");
+        CK_HIP(hipEventRecord(l.done, l.stream), "hipEventRecord");
     }
     l.busy = true;
     l.seq = s->next_seq++;
@@ -403,16 +418,17 @@ extern "C" int32_t ck_serve_destroy(ck_serve* s) {
     if (s == nullptr) return CK_OK;
     int32_t st = CK_OK;
     if (!s->dry) {
-        bool any = false;
-        for (const Launch& l : s->launches) any = any || l.busy;
-        if (any && s->stream != nullptr)
-            st = ck::hip_status(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
-        else if (any)
-            st = ck::hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize");
         for (Launch& l : s->launches) {
+            if (l.stream) {
+                const int32_t e = ck::hip_status(hipStreamSynchronize(l.stream),
+                                                 "hipStreamSynchronize");
+                st = st != CK_OK ? st : e;
+                (void)hipStreamDestroy(l.stream);
+            }
             if (l.done) (void)hipEventDestroy(l.done);
             if (l.host_desc) (void)hipHostFree(l.host_desc);
         }
+        if (s->inputs) (void)hipEventDestroy(s->inputs);
     }
     delete s;
     return st;

@@ -299,7 +299,8 @@ int32_t ck_serve_admit(ck_serve* server, int32_t code, int64_t frames, uint64_t 
                        ck_serve_slot* where);
 
 /* The request's LLRs are in place (written on, or ordered before work on,
- * the engine's stream). */
+ * the stream given to ck_serve_create; launches wait for that stream's
+ * work, then run on streams of their own). */
 int32_t ck_serve_submit(ck_serve* server, uint64_t tag);
 
 /* Launches every submitted request not yet launched (up to 8 contiguous
