@@ -21,11 +21,13 @@
 //   qc_layered  quasi-cyclic codes.  A workgroup decodes F frames whose
 //               APPs sit in LDS; lane (f, r, p) owns row r of the current
 //               block row for frame f, with P lanes per row splitting its
-//               edges and combining min1/min2/idx/sign with DPP quad moves
-//               (an exact, order-independent reduction on (|Q|, edge)
-//               pairs; for sum-product the phi sum and sign parity).  Columns come from the circulant shift,
-//               (r + s) mod Z, so consecutive lanes touch consecutive LDS
-//               words.  Block rows run in schedule order (a layer is a union
+//               edges and combining min1/min2/idx/sign across the row's
+//               lanes (P = 2: the two halves of a wave, one
+//               v_permlane32_swap per value; P = 4: DPP quad moves) -- an
+//               exact, order-independent reduction on (|Q|, edge) pairs; for
+//               sum-product the phi sum and sign parity.  Columns come from
+//               the circulant shift, (r + s) mod Z, so consecutive lanes
+//               (of a half-wave for P = 2) touch consecutive LDS words.  Block rows run in schedule order (a layer is a union
 //               of whole block rows, whose rows share no variable), with a
 //               barrier after each.  Min-sum keeps each check row's state
 //               compressed to {m1, m2, sign bits, idx} (16 bytes instead of

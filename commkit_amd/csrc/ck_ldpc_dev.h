@@ -131,11 +131,22 @@ struct QcLane {
     bool live;
 };
 
+// The lanes of one check row: P = 2 puts them in lanes i and i + 32 of a
+// wave, so each half-wave reads 32 consecutive rows of a circulant (no LDS
+// bank conflicts between the two halves' columns); P = 4 puts them in a quad.
+// Workgroups are whole waves, so both halves exist.
 template <int P>
 __device__ __forceinline__ QcLane qc_lane(int Z, int F, long long R) {
     QcLane l;
-    const int row_lane = threadIdx.x / P;
-    l.p = threadIdx.x - row_lane * P;
+    int row_lane;
+    if constexpr (P == 2) {
+        const int lane = static_cast<int>(threadIdx.x & 63);
+        row_lane = static_cast<int>((threadIdx.x >> 6) << 5) | (lane & 31);
+        l.p = lane >> 5;
+    } else {
+        row_lane = threadIdx.x / P;
+        l.p = threadIdx.x - row_lane * P;
+    }
     l.f = row_lane / Z;
     l.r = row_lane - l.f * Z;
     l.u = l.r * P + l.p;
@@ -196,6 +207,17 @@ __device__ __forceinline__ float& lds_at(float* ar, int off) {
 __device__ __forceinline__ int lane_xor(int v, int off) {
     return off == 1 ? __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false)   // quad_perm 1,0,3,2
                     : __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+}
+
+// Both values of a two-lane check row (lanes i and i + 32) in each of its
+// lanes: .x from the lane with p = 0, .y from p = 1.  v_permlane32_swap
+// exchanges the upper half of its first operand with the lower half of its
+// second; with v in both, the first result holds the lower half's values in
+// every lane and the second the upper half's.
+__device__ __forceinline__ int2 row_pair(int v) {
+    const auto s = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(v),
+                                                    static_cast<unsigned>(v), false, false);
+    return make_int2(static_cast<int>(s[0]), static_cast<int>(s[1]));
 }
 
 // `mag` (sign bit clear) with the sign of bit e of `bits`.
@@ -407,18 +429,39 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
         min1 = fminf(min1, mag);
         qneg |= (valid && qv < 0.0f) ? (1u << e) : 0u;
     }
-#pragma unroll
-    for (int off = 1; off < P; off <<= 1) {
-        const float o1 = __int_as_float(lane_xor(__float_as_int(min1), off));
-        const float o2 = __int_as_float(lane_xor(__float_as_int(min2), off));
-        const int oi = lane_xor(idx, off);
-        qneg |= static_cast<unsigned>(lane_xor(static_cast<int>(qneg), off));
-        if (o1 < min1 || (o1 == min1 && oi < idx)) {
-            min2 = fminf(min1, o2);
-            min1 = o1;
-            idx = oi;
+    // Lanes of a row merge their (min1, min2, idx) by the same rule, which
+    // gives the same result whichever side is taken as the lane's own.
+    if constexpr (P == 2) {
+        const int2 v1 = row_pair(__float_as_int(min1));
+        const int2 v2 = row_pair(__float_as_int(min2));
+        const int2 vi = row_pair(idx);
+        const int2 vq = row_pair(static_cast<int>(qneg));
+        const float x1 = __int_as_float(v1.x);
+        const float y1 = __int_as_float(v1.y);
+        if (y1 < x1 || (y1 == x1 && vi.y < vi.x)) {
+            min2 = fminf(x1, __int_as_float(v2.y));
+            min1 = y1;
+            idx = vi.y;
         } else {
-            min2 = fminf(min2, o1);
+            min2 = fminf(__int_as_float(v2.x), y1);
+            min1 = x1;
+            idx = vi.x;
+        }
+        qneg = static_cast<unsigned>(vq.x | vq.y);
+    } else {
+#pragma unroll
+        for (int off = 1; off < P; off <<= 1) {
+            const float o1 = __int_as_float(lane_xor(__float_as_int(min1), off));
+            const float o2 = __int_as_float(lane_xor(__float_as_int(min2), off));
+            const int oi = lane_xor(idx, off);
+            qneg |= static_cast<unsigned>(lane_xor(static_cast<int>(qneg), off));
+            if (o1 < min1 || (o1 == min1 && oi < idx)) {
+                min2 = fminf(min1, o2);
+                min1 = o1;
+                idx = oi;
+            } else {
+                min2 = fminf(min2, o1);
+            }
         }
     }
     const float m1 = fmaxf(prm.scale * min1 - prm.offset, 0.0f);
@@ -526,10 +569,17 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
         total = total + ph[t];
         neg ^= (valid && q[t] < 0.0f) ? 1u : 0u;
     }
+    if constexpr (P == 2) {
+        const int2 vt = row_pair(__float_as_int(total));
+        const int2 vn = row_pair(static_cast<int>(neg));
+        total = __int_as_float(vt.x) + __int_as_float(vt.y);
+        neg = static_cast<unsigned>(vn.x ^ vn.y);
+    } else {
 #pragma unroll
-    for (int off = 1; off < P; off <<= 1) {
-        total = total + __int_as_float(lane_xor(__float_as_int(total), off));
-        neg ^= static_cast<unsigned>(lane_xor(static_cast<int>(neg), off));
+        for (int off = 1; off < P; off <<= 1) {
+            total = total + __int_as_float(lane_xor(__float_as_int(total), off));
+            neg ^= static_cast<unsigned>(lane_xor(static_cast<int>(neg), off));
+        }
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {

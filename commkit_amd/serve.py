@@ -338,13 +338,18 @@ class LoadResult:
 
 def drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None], *,
           clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] | None = None,
-          idle_s: float = 0.0) -> LoadResult:
+          idle_s: float = 0.0, on_done: Callable[[Slot, int], None] | None = None,
+          admit_slice_s: float = 1e-3) -> LoadResult:
     """Run ``schedule`` against ``server``: each request is admitted when it
     has arrived (or, when the budget is full, as soon as room frees up), its
     LLRs written by ``fill(slot, request index)``, submitted, and launched by
-    the next :meth:`Server.step`; completions are polled every loop turn and
-    released at once.  Returns per-frame latencies (completion seen minus
-    arrival)."""
+    the next :meth:`Server.step`; completions are polled every loop turn,
+    handed to ``on_done(slot, request index)`` if given (work it enqueues on
+    the server's stream reads the results before the slot can be reused) and
+    released.  A loop turn admits for at most ``admit_slice_s`` seconds, so a
+    long backlog is launched in pieces while the rest is still being admitted
+    instead of leaving the device idle.  Returns per-frame latencies
+    (completion seen minus arrival)."""
     n = len(schedule)
     t_arr = schedule.t
     t_done = np.full(n, np.nan)
@@ -363,7 +368,8 @@ def drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None],
         # First come, first served: admission stops at the first request that
         # does not fit, so a large request is not overtaken indefinitely.
         k = 0
-        while k < len(backlog):
+        turn_end = clock() + admit_slice_s
+        while k < len(backlog) and (k == 0 or clock() < turn_end):
             i = backlog[k]
             slot = server.admit(int(schedule.code[i]), int(schedule.frames[i]), i)
             if slot is None:
@@ -379,6 +385,8 @@ def drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None],
             launches += 1
         for tag in server.poll():
             t_done[tag] = clock() - start
+            if on_done is not None:
+                on_done(server.slot(tag), tag)
             server.release(tag)
             done += 1
         if sleep is not None and not backlog and nxt < n and done == nxt:
