@@ -55,6 +55,7 @@ class CkBatchPlan(ctypes.Structure):
         ("lds_bytes", ctypes.c_int32),
         ("sum_product", ctypes.c_int32),
         ("frames_per_group", ctypes.c_int32 * 8),
+        ("lanes_per_row", ctypes.c_int32 * 8),
     ]
 
 

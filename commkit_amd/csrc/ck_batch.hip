@@ -73,7 +73,7 @@ __global__ void batch_reset(BatchDesc* d) {
 // queue is empty), loads the frame's LLRs, and with early stop retires at
 // once the frames already satisfied (0 iterations), asking again.
 template <int P>
-__device__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_code& c,
+__device__ __forceinline__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_code& c,
                        const int2* tab, float* lds, unsigned* bits, float* a, const QcLane& l,
                        bool lane_ok, int F,
                        const Params& prm, int* s_need, int* s_frame, int* s_flag, bool& active,
@@ -91,8 +91,7 @@ __device__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_co
         const bool fresh = lane_ok && s_need[l.f] && s_frame[l.f] >= 0;
         if (fresh) {
             frame = s_frame[l.f];
-            const float* src = bc.llrs + frame * n;
-            for (int v = l.u; v < N; v += Z * P) a[v] = v < n ? src[v] : 0.0f;
+            load_frame(a, bc.llrs + frame * n, n, N, l.u, Z * P);
         }
         __syncthreads();
         if (lane_ok && l.u == 0) s_need[l.f] = 0;
@@ -112,7 +111,7 @@ __device__ void refill(const BatchClass& bc, unsigned* counter, const ck_ldpc_co
 }
 
 template <bool SP, int P, bool STATE_LDS>
-__device__ __attribute__((noinline)) void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s_tab, int* s_need,
+__device__ __forceinline__ void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s_tab, int* s_need,
                           int* s_frame, int* s_flag) {
     const BatchClass bc = d->cls[ci];
     const ck_ldpc_code c = bc.code;
@@ -148,7 +147,7 @@ __device__ __attribute__((noinline)) void run_class(BatchDesc* __restrict__ d, i
             if (active) {
                 const bool first = my_it == 0;
                 if (SP) {
-                    qc_sp_row<1, true>(c, s_tab, a, Rm, br, l.r, 0, first);
+                    qc_sp_row<P, true>(c, s_tab, a, Rm, br, l.r, l.p, first);
                 } else {
                     const CheckState st = first ? zero_state() : st_next;
                     if (br + 1 < mb) {
@@ -177,7 +176,12 @@ __device__ __attribute__((noinline)) void run_class(BatchDesc* __restrict__ d, i
     }
 }
 
-template <bool SP>
+// P1: some class runs one lane per check row.  Its rows take up to
+// kMaxDegree slots of a lane, which costs registers (spills) that the
+// two-lane classes should not pay, so that path is compiled into its own
+// kernel.  Everything is inlined into the kernel: calls would move the live
+// registers to scratch memory around every row and syndrome check.
+template <bool SP, bool P1>
 __global__ __launch_bounds__(kBatchThreads) void ldpc_batch(BatchDesc* __restrict__ d) {
     extern __shared__ float lds[];
     __shared__ int2 s_tab[kTabEntries];
@@ -186,16 +190,17 @@ __global__ __launch_bounds__(kBatchThreads) void ldpc_batch(BatchDesc* __restric
     __shared__ int s_flag[kMaxFrames];
     const int nc = d->num_classes;
     for (int ci = 0; ci < nc; ++ci) {
-        if (d->cls[ci].P == 2)
+        if (d->cls[ci].P == 2) {
             if (!SP && d->cls[ci].state_lds)
                 run_class<SP, 2, true>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
             else
                 run_class<SP, 2, false>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
-        else
+        } else if constexpr (P1) {
             if (!SP && d->cls[ci].state_lds)
                 run_class<SP, 1, true>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
             else
                 run_class<SP, 1, false>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+        }
     }
 }
 
@@ -244,7 +249,6 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
         const size_t app_bytes = qc_lds(1, c.num_variables, c.lifting, c.num_checks, false).end * 4;
         int P = bc.lanes_per_row;
         if (P == 0) P = (2 * c.lifting <= kBatchThreads) ? 2 : 1;  // twice the lanes per frame
-        if (sp) P = 1;
         CK_REQUIRE(P == 1 || P == 2, "class %d: lanes_per_row %d not 1 or 2", i, P);
         CK_REQUIRE(c.lifting * P <= kBatchThreads, "class %d: lifting %d x %d lanes above %d", i,
                    c.lifting, P, kBatchThreads);
@@ -323,24 +327,39 @@ void describe(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
     }
 }
 
-void allow_lds(bool sum_product, size_t lds_bytes) {
+bool one_lane(const Shape& shape, int32_t num_classes) {
+    for (int i = 0; i < num_classes; ++i)
+        if (shape.P[i] == 1) return true;
+    return false;
+}
+
+void allow_lds(size_t lds_bytes) {
     if (lds_bytes > 65536) {
-        const void* k = sum_product ? reinterpret_cast<const void*>(ldpc_batch<true>)
-                                    : reinterpret_cast<const void*>(ldpc_batch<false>);
-        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds_bytes));
+        const void* kernels[] = {reinterpret_cast<const void*>(ldpc_batch<false, false>),
+                                 reinterpret_cast<const void*>(ldpc_batch<false, true>),
+                                 reinterpret_cast<const void*>(ldpc_batch<true, false>),
+                                 reinterpret_cast<const void*>(ldpc_batch<true, true>)};
+        for (const void* k : kernels)
+            (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds_bytes));
         (void)hipGetLastError();
     }
 }
 
-int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, hipStream_t s) {
+int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, bool one_lane_rows,
+               hipStream_t s) {
     BatchDesc* d = static_cast<BatchDesc*>(workspace);
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, d);
     if (int32_t st = ck::launched("batch reset")) return st;
-    if (sum_product)
-        hipLaunchKernelGGL(ldpc_batch<true>, dim3(grid), dim3(kBatchThreads), lds_bytes, s, d);
+    const dim3 g(static_cast<unsigned>(grid)), b(kBatchThreads);
+    if (sum_product && one_lane_rows)
+        hipLaunchKernelGGL((ldpc_batch<true, true>), g, b, lds_bytes, s, d);
+    else if (sum_product)
+        hipLaunchKernelGGL((ldpc_batch<true, false>), g, b, lds_bytes, s, d);
+    else if (one_lane_rows)
+        hipLaunchKernelGGL((ldpc_batch<false, true>), g, b, lds_bytes, s, d);
     else
-        hipLaunchKernelGGL(ldpc_batch<false>, dim3(grid), dim3(kBatchThreads), lds_bytes, s, d);
+        hipLaunchKernelGGL((ldpc_batch<false, false>), g, b, lds_bytes, s, d);
     return ck::launched("batch decode");
 }
 
@@ -376,21 +395,26 @@ extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_c
     ckbatch::describe(classes, num_classes, method, shp, workspace, &d);
     CK_HIP(hipMemcpy(workspace, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy(batch descriptor)");
     // Set here, outside any stream capture: the launch itself only enqueues.
-    ckbatch::allow_lds(d.sum_product != 0, L.lds_bytes);
+    ckbatch::allow_lds(L.lds_bytes);
     plan->workspace = workspace;
     plan->grid = L.grid;
     plan->threads = kBatchThreads;
     plan->lds_bytes = static_cast<int32_t>(L.lds_bytes);
     plan->sum_product = d.sum_product;
-    for (int i = 0; i < kMaxClasses; ++i) plan->frames_per_group[i] = i < num_classes ? L.F[i] : 0;
+    for (int i = 0; i < kMaxClasses; ++i) {
+        plan->frames_per_group[i] = i < num_classes ? L.F[i] : 0;
+        plan->lanes_per_row[i] = i < num_classes ? L.P[i] : 0;
+    }
     return CK_OK;
 }
 
 extern "C" int32_t ck_batch_launch(const ck_batch_plan* plan, void* stream) {
     CK_REQUIRE(plan != nullptr && plan->workspace != nullptr, "plan is not prepared");
     CK_REQUIRE(plan->grid >= 1 && plan->threads == kBatchThreads, "plan is not prepared");
+    bool one_lane_rows = false;
+    for (int i = 0; i < kMaxClasses; ++i) one_lane_rows = one_lane_rows || plan->lanes_per_row[i] == 1;
     return ckbatch::launch(plan->workspace, plan->grid, static_cast<size_t>(plan->lds_bytes),
-                           plan->sum_product != 0, static_cast<hipStream_t>(stream));
+                           plan->sum_product != 0, one_lane_rows, static_cast<hipStream_t>(stream));
 }
 
 namespace {

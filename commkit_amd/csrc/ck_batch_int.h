@@ -34,11 +34,16 @@ int32_t shape(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
 void describe(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_method* method,
               const Shape& shape, void* workspace, void* host_desc);
 
-// Raises the kernel's dynamic LDS limit when needed (not under capture).
-void allow_lds(bool sum_product, size_t lds_bytes);
+// Whether any of the first `num_classes` classes runs one lane per row
+// (those launches use the kernel compiled with that path).
+bool one_lane(const Shape& shape, int32_t num_classes);
+
+// Raises the kernels' dynamic LDS limit when needed (not under capture).
+void allow_lds(size_t lds_bytes);
 
 // Clears the queue counters and launches the decode on `stream`; the
 // descriptor must be in `workspace` by then (stream order).
-int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, hipStream_t stream);
+int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, bool one_lane_rows,
+               hipStream_t stream);
 
 }  // namespace ckbatch

@@ -54,10 +54,14 @@ namespace {
 
 using namespace ckdev;
 
-// SHORT (min-sum, rows of at most kShortRow edges): row instantiations up
-// to kShortRow / P slots, with a register budget of 168 VGPRs (SHORT = 1:
-// one workgroup of up to twelve waves per CU) or 128 (SHORT = 2, P = 1: two
-// workgroups of up to six waves).
+// SHORT (min-sum, short rows): row instantiations up to kShortRow / P slots
+// with a register budget of 168 VGPRs (SHORT = 1: one workgroup of up to
+// twelve waves per CU), or up to kShortRowTwo slots with 128 (SHORT = 2,
+// P = 1: two workgroups of up to six waves, whose barriers and frame
+// loads and stores overlap).
+template <int P, int SHORT>
+constexpr int kRowSlots = SHORT == 2 ? kShortRowTwo / P : (SHORT == 1 ? kShortRow / P : kMaxDegree / P);
+
 template <bool SP, int P, bool STATE_LDS, int SHORT>
 __global__ __launch_bounds__(kMaxQcThreads)
 __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void ldpc_qc_layered(
@@ -125,7 +129,7 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
                         st_next = st_base[l.r];  // block row 0, stored this iteration
                     }
                     const CheckState ns =
-                        qc_ms_row<P, true, SHORT != 0 ? kShortRow / P : kMaxDegree / P>(c, s_tab, a, st, br, l.r, l.p, prm);
+                        qc_ms_row<P, true, kRowSlots<P, SHORT>>(c, s_tab, a, st, br, l.r, l.p, prm);
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
                 }
@@ -477,7 +481,7 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, P
         p.short_rows = 0;
         if (!p.sp && !p.flood && P <= 2 && c->max_row_degree <= kShortRow) {
             // Two workgroups per CU when at most six waves each and LDS allows.
-            const bool two = P == 1 && p.threads <= 384 &&
+            const bool two = P == 1 && p.threads <= 384 && c->max_row_degree <= kShortRowTwo &&
                              2 * (p.lds_bytes + kStaticLds) <= info.lds_per_block;
             p.short_rows = two ? 2 : 1;
         }

@@ -73,11 +73,49 @@ __device__ __forceinline__ float state_msg(const CheckState& st, int e) {
 }
 
 // Channel LLRs of frames [frame0, frame0 + F) into LDS (zeros for the
-// punctured tail and for frames past R).
+// punctured tail and for frames past R).  The 16-byte path walks the F
+// frames as one range of F * N/4 vectors and issues up to kLoadBatch of a
+// lane's loads before the first LDS write, so the workgroup waits for a
+// couple of HBM round trips instead of one per vector.
+constexpr int kLoadBatch = 8;
+
 __device__ inline void load_frames(float* lds, const float* __restrict__ llr, long long frame0, int F,
                             long long R, int n, int N) {
     const bool vec = ((n & 3) == 0) && ((N & 3) == 0) &&
                      ((reinterpret_cast<uintptr_t>(llr) & 15) == 0);
+    if (vec) {
+        const int n4 = N >> 2;          // vectors per frame in LDS
+        const int m4 = n >> 2;          // of them read from the frame
+        const int total = F * n4;
+        const int step = static_cast<int>(blockDim.x);
+        float4* d4 = reinterpret_cast<float4*>(lds);
+        // (f, j): frame and vector of this lane's next element q.
+        int q = static_cast<int>(threadIdx.x);
+        int f = q / n4;
+        int j = q - f * n4;
+        while (q < total) {
+            float4 v[kLoadBatch];
+            int at[kLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u) {
+                at[u] = q;
+                const long long frame = frame0 + f;
+                const bool load = q < total && j < m4 && frame < R;
+                v[u] = load ? reinterpret_cast<const float4*>(llr + frame * n)[j]
+                            : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                q += step;
+                j += step;
+                while (j >= n4) {
+                    j -= n4;
+                    ++f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (at[u] < total) d4[at[u]] = v[u];
+        }
+        return;
+    }
     for (int f = 0; f < F; ++f) {
         const long long frame = frame0 + f;
         float* dst = lds + static_cast<size_t>(f) * N;
@@ -86,13 +124,47 @@ __device__ inline void load_frames(float* lds, const float* __restrict__ llr, lo
             continue;
         }
         const float* src = llr + frame * n;
-        if (vec) {
-            const float4* s4 = reinterpret_cast<const float4*>(src);
-            float4* d4 = reinterpret_cast<float4*>(dst);
-            for (int q = threadIdx.x; q < N / 4; q += blockDim.x)
-                d4[q] = (4 * q < n) ? s4[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        } else {
-            for (int v = threadIdx.x; v < N; v += blockDim.x) dst[v] = v < n ? src[v] : 0.0f;
+        for (int v = threadIdx.x; v < N; v += blockDim.x) dst[v] = v < n ? src[v] : 0.0f;
+    }
+}
+
+// Channel LLRs of one frame (n of them at src, zeros up to N) into `a`,
+// by the `lanes` lanes numbered u = 0..lanes-1 of that frame; 16-byte
+// vectors when the row allows, and up to kLoadBatch loads of a lane in
+// flight before its LDS writes.
+__device__ inline void load_frame(float* a, const float* __restrict__ src, int n, int N, int u,
+                                  int lanes) {
+    if (((n & 3) == 0) && ((N & 3) == 0) && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+        const int n4 = N >> 2;
+        const int m4 = n >> 2;
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        float4* d4 = reinterpret_cast<float4*>(a);
+        for (int j0 = u; j0 < n4; j0 += kLoadBatch * lanes) {
+            float4 v[kLoadBatch];
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int j = j0 + k * lanes;
+                v[k] = j < m4 ? s4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int j = j0 + k * lanes;
+                if (j < n4) d4[j] = v[k];
+            }
+        }
+        return;
+    }
+    for (int v0 = u; v0 < N; v0 += kLoadBatch * lanes) {
+        float x[kLoadBatch];
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; ++k) {
+            const int v = v0 + k * lanes;
+            x[k] = v < n ? src[v] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; ++k) {
+            const int v = v0 + k * lanes;
+            if (v < N) a[v] = x[k];
         }
     }
 }
@@ -103,13 +175,38 @@ __device__ inline void store_frame(const ck_ldpc_code& c, const float* a, long l
                             int lanes, int it, bool ok, float* __restrict__ app_out,
                             signed char* __restrict__ bits_out, int* __restrict__ iters_out,
                             unsigned char* __restrict__ valid_out) {
+    // kLoadBatch LDS reads of a lane in flight before its stores.
     float* dst = app_out + frame * c.n;
-    for (int v = u; v < c.n; v += lanes) dst[v] = a[v];
+    for (int v0 = u; v0 < c.n; v0 += kLoadBatch * lanes) {
+        float x[kLoadBatch];
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; ++k) {
+            const int v = v0 + k * lanes;
+            x[k] = v < c.n ? a[v] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; ++k) {
+            const int v = v0 + k * lanes;
+            if (v < c.n) dst[v] = x[k];
+        }
+    }
     if (bits_out != nullptr) {
         signed char* b = bits_out + frame * c.k;
-        for (int i = u; i < c.k; i += lanes) {
-            const int pos = c.info ? c.info[i] : i;
-            b[i] = a[pos] < 0.0f ? 1 : 0;
+        for (int i0 = u; i0 < c.k; i0 += kLoadBatch * lanes) {
+            int pos[kLoadBatch];
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int i = i0 + k * lanes;
+                pos[k] = i < c.k ? (c.info ? c.info[i] : i) : 0;
+            }
+            bool neg[kLoadBatch];
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) neg[k] = a[pos[k]] < 0.0f;
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int i = i0 + k * lanes;
+                if (i < c.k) b[i] = neg[k] ? 1 : 0;
+            }
         }
     }
     if (u == 0) {
@@ -236,7 +333,7 @@ __device__ __forceinline__ float with_sign(float mag, unsigned bits, int e) {
 // each task (slot, block row, 32 rows from r0) XORs, over the block row's
 // circulants {j, shift}, the 32 bits from column (r0 + shift) mod Z of
 // bitmap j: bit i of the result is the parity of row r0 + i.
-__device__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const float* app,
+__device__ __forceinline__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const float* app,
                                unsigned* bits, const QcLane& l, bool lane_ok, bool test, int F,
                                int* s_flag) {
     constexpr int U = 8;  // loads in flight per lane
@@ -502,9 +599,13 @@ __device__ __forceinline__ CheckState qc_ms_dispatch(int groups, const int2* tab
     }
 }
 
-// Slots per lane of the kernels for rows of up to kShortRow edges (P = 1):
-// fewer registers, so two workgroups of six waves share a CU.
+// Slots per lane of the kernels for short rows (P = 1; halved for P = 2):
+// fewer registers than rows of up to kMaxDegree edges.  Up to kShortRow
+// edges the kernel fits 168 VGPRs (one workgroup of twelve waves per CU);
+// up to kShortRowTwo it fits 128 without spilling, so two workgroups of six
+// waves share a CU.
 constexpr int kShortRow = 24;
+constexpr int kShortRowTwo = 22;
 
 // One min-sum check row `r` of block row `br`; MAXT bounds the lane's slots
 // (ceil(row degree / P)), which the host checks before choosing the kernel.

@@ -201,7 +201,7 @@ extern "C" int32_t ck_serve_create(const ck_serve_config* config, void* const* w
             st = ckbatch::shape(&c, 1, config->method, &sh);
             lds = std::max(lds, sh.lds_bytes);
         }
-        if (st == CK_OK) ckbatch::allow_lds(config->method->rule == 1, lds);
+        if (st == CK_OK) ckbatch::allow_lds(lds);
         for (size_t w = 0; w < s->launches.size() && st == CK_OK; ++w) {
             Launch& l = s->launches[w];
             l.workspace = workspaces[w];
@@ -356,7 +356,7 @@ extern "C" int32_t ck_serve_step(ck_serve* s, int64_t* launched_frames) {
                               hipMemcpyHostToDevice, l.stream),
                "hipMemcpyAsync(descriptor)");
         if (int32_t st = ckbatch::launch(l.workspace, sh.grid, sh.lds_bytes, s->method.rule == 1,
-                                         l.stream))
+                                         ckbatch::one_lane(sh, n), l.stream))
             return st;
         CK_HIP(hipEventRecord(l.done, l.stream), "hipEventRecord");
     }

@@ -30,9 +30,11 @@ def _llrs(code, rows, seed):
 
 
 def _alone(code, x, method):
-    # The batched decoder runs sum-product with one lane per row; the per-call
-    # reference must add a row's phi terms in the same order (P = 1).
-    tiles = (2, 0, 1, 0) if isinstance(method, SumProduct) else None
+    # The batched decoder runs two lanes per check row when two rows' lanes
+    # fit its 768-lane workgroup, else one; for sum-product the per-call
+    # reference must add a row's phi terms in the same order (same P).
+    lanes = 2 if 2 * code._qc_view["lifting"] <= 768 else 1
+    tiles = (2, 0, lanes, 0) if isinstance(method, SumProduct) else None
     bits, app, valid, iters = _decode(code, x, method, tiles=tiles)
     return [t.cpu().numpy() for t in (bits, app, valid.to(dtype=bits.dtype), iters)]
 
