@@ -56,6 +56,7 @@ class CkBatchPlan(ctypes.Structure):
         ("sum_product", ctypes.c_int32),
         ("frames_per_group", ctypes.c_int32 * 8),
         ("lanes_per_row", ctypes.c_int32 * 8),
+        ("kernel_variant", ctypes.c_int32),
     ]
 
 

@@ -110,7 +110,7 @@ __device__ __forceinline__ void refill(const BatchClass& bc, unsigned* counter, 
     }
 }
 
-template <bool SP, int P, bool STATE_LDS>
+template <bool SP, int P, bool STATE_LDS, int MAXT>
 __device__ __forceinline__ void run_class(BatchDesc* __restrict__ d, int ci, float* lds, int2* s_tab, int* s_need,
                           int* s_frame, int* s_flag) {
     const BatchClass bc = d->cls[ci];
@@ -147,7 +147,7 @@ __device__ __forceinline__ void run_class(BatchDesc* __restrict__ d, int ci, flo
             if (active) {
                 const bool first = my_it == 0;
                 if (SP) {
-                    qc_sp_row<P, true>(c, s_tab, a, Rm, br, l.r, l.p, first);
+                    qc_sp_row<P, true, MAXT>(c, s_tab, a, Rm, br, l.r, l.p, first);
                 } else {
                     const CheckState st = first ? zero_state() : st_next;
                     if (br + 1 < mb) {
@@ -155,7 +155,7 @@ __device__ __forceinline__ void run_class(BatchDesc* __restrict__ d, int ci, flo
                     } else if (mb > 1) {
                         st_next = st_base[l.r];  // block row 0, stored this iteration
                     }
-                    const CheckState ns = qc_ms_row<P, true>(c, s_tab, a, st, br, l.r, l.p, prm);
+                    const CheckState ns = qc_ms_row<P, true, MAXT>(c, s_tab, a, st, br, l.r, l.p, prm);
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
                 }
@@ -176,13 +176,16 @@ __device__ __forceinline__ void run_class(BatchDesc* __restrict__ d, int ci, flo
     }
 }
 
-// P1: some class runs one lane per check row.  Its rows take up to
-// kMaxDegree slots of a lane, which costs registers (spills) that the
-// two-lane classes should not pay, so that path is compiled into its own
-// kernel.  Everything is inlined into the kernel: calls would move the live
-// registers to scratch memory around every row and syndrome check.
-template <bool SP, bool P1>
+// ONE: whether some class runs one lane per check row (0: none; 1: all such
+// classes have rows of at most kShortRow edges; 2: longer rows).  A lane of
+// a one-lane row holds all its slots in registers, which costs the kernel
+// registers (spills) that two-lane classes (rows of up to kMaxDegree / 2
+// slots a lane) should not pay, so each case is its own kernel.  Everything
+// is inlined: calls would move the live registers to scratch memory around
+// every row and syndrome check.
+template <bool SP, int ONE>
 __global__ __launch_bounds__(kBatchThreads) void ldpc_batch(BatchDesc* __restrict__ d) {
+    constexpr int kOneLaneSlots = ONE == 1 ? kShortRow : kMaxDegree;
     extern __shared__ float lds[];
     __shared__ int2 s_tab[kTabEntries];
     __shared__ int s_need[kMaxFrames];
@@ -192,14 +195,14 @@ __global__ __launch_bounds__(kBatchThreads) void ldpc_batch(BatchDesc* __restric
     for (int ci = 0; ci < nc; ++ci) {
         if (d->cls[ci].P == 2) {
             if (!SP && d->cls[ci].state_lds)
-                run_class<SP, 2, true>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+                run_class<SP, 2, true, kMaxDegree / 2>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
             else
-                run_class<SP, 2, false>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
-        } else if constexpr (P1) {
+                run_class<SP, 2, false, kMaxDegree / 2>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+        } else if constexpr (ONE != 0) {
             if (!SP && d->cls[ci].state_lds)
-                run_class<SP, 1, true>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+                run_class<SP, 1, true, kOneLaneSlots>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
             else
-                run_class<SP, 1, false>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+                run_class<SP, 1, false, kOneLaneSlots>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
         }
     }
 }
@@ -211,6 +214,7 @@ struct Layout {
     int F[kMaxClasses];
     int P[kMaxClasses];
     int S[kMaxClasses];  // check states in LDS
+    int one_lane;        // kernel variant: one-lane rows (0 none, 1 short, 2 long)
 };
 
 int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_method* m,
@@ -273,6 +277,10 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
         L.slot_bytes = std::max(L.slot_bytes, static_cast<size_t>(F) * per_slot);
     }
     L.slot_bytes = (L.slot_bytes + 255) / 256 * 256;
+    L.one_lane = 0;
+    for (int i = 0; i < num_classes; ++i)
+        if (L.P[i] == 1)
+            L.one_lane = std::max(L.one_lane, classes[i].code->max_row_degree <= kShortRow ? 1 : 2);
     // One workgroup of twelve waves per CU: its registers (up to 168 VGPRs
     // a lane) leave no room for a second.
     L.grid = info.num_cus;
@@ -298,6 +306,7 @@ int32_t shape(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
         out->P[i] = L.P[i];
         out->S[i] = L.S[i];
     }
+    out->one_lane = L.one_lane;
     return CK_OK;
 }
 
@@ -327,18 +336,14 @@ void describe(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
     }
 }
 
-bool one_lane(const Shape& shape, int32_t num_classes) {
-    for (int i = 0; i < num_classes; ++i)
-        if (shape.P[i] == 1) return true;
-    return false;
-}
-
 void allow_lds(size_t lds_bytes) {
     if (lds_bytes > 65536) {
-        const void* kernels[] = {reinterpret_cast<const void*>(ldpc_batch<false, false>),
-                                 reinterpret_cast<const void*>(ldpc_batch<false, true>),
-                                 reinterpret_cast<const void*>(ldpc_batch<true, false>),
-                                 reinterpret_cast<const void*>(ldpc_batch<true, true>)};
+        const void* kernels[] = {reinterpret_cast<const void*>(ldpc_batch<false, 0>),
+                                 reinterpret_cast<const void*>(ldpc_batch<false, 1>),
+                                 reinterpret_cast<const void*>(ldpc_batch<false, 2>),
+                                 reinterpret_cast<const void*>(ldpc_batch<true, 0>),
+                                 reinterpret_cast<const void*>(ldpc_batch<true, 1>),
+                                 reinterpret_cast<const void*>(ldpc_batch<true, 2>)};
         for (const void* k : kernels)
             (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(lds_bytes));
@@ -346,20 +351,22 @@ void allow_lds(size_t lds_bytes) {
     }
 }
 
-int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, bool one_lane_rows,
+int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, int one_lane,
                hipStream_t s) {
+    CK_REQUIRE(one_lane >= 0 && one_lane <= 2, "kernel variant %d", one_lane);
     BatchDesc* d = static_cast<BatchDesc*>(workspace);
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, d);
     if (int32_t st = ck::launched("batch reset")) return st;
     const dim3 g(static_cast<unsigned>(grid)), b(kBatchThreads);
-    if (sum_product && one_lane_rows)
-        hipLaunchKernelGGL((ldpc_batch<true, true>), g, b, lds_bytes, s, d);
-    else if (sum_product)
-        hipLaunchKernelGGL((ldpc_batch<true, false>), g, b, lds_bytes, s, d);
-    else if (one_lane_rows)
-        hipLaunchKernelGGL((ldpc_batch<false, true>), g, b, lds_bytes, s, d);
-    else
-        hipLaunchKernelGGL((ldpc_batch<false, false>), g, b, lds_bytes, s, d);
+    if (sum_product) {
+        if (one_lane == 0) hipLaunchKernelGGL((ldpc_batch<true, 0>), g, b, lds_bytes, s, d);
+        else if (one_lane == 1) hipLaunchKernelGGL((ldpc_batch<true, 1>), g, b, lds_bytes, s, d);
+        else hipLaunchKernelGGL((ldpc_batch<true, 2>), g, b, lds_bytes, s, d);
+    } else {
+        if (one_lane == 0) hipLaunchKernelGGL((ldpc_batch<false, 0>), g, b, lds_bytes, s, d);
+        else if (one_lane == 1) hipLaunchKernelGGL((ldpc_batch<false, 1>), g, b, lds_bytes, s, d);
+        else hipLaunchKernelGGL((ldpc_batch<false, 2>), g, b, lds_bytes, s, d);
+    }
     return ck::launched("batch decode");
 }
 
@@ -405,16 +412,16 @@ extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_c
         plan->frames_per_group[i] = i < num_classes ? L.F[i] : 0;
         plan->lanes_per_row[i] = i < num_classes ? L.P[i] : 0;
     }
+    plan->kernel_variant = L.one_lane;
     return CK_OK;
 }
 
 extern "C" int32_t ck_batch_launch(const ck_batch_plan* plan, void* stream) {
     CK_REQUIRE(plan != nullptr && plan->workspace != nullptr, "plan is not prepared");
     CK_REQUIRE(plan->grid >= 1 && plan->threads == kBatchThreads, "plan is not prepared");
-    bool one_lane_rows = false;
-    for (int i = 0; i < kMaxClasses; ++i) one_lane_rows = one_lane_rows || plan->lanes_per_row[i] == 1;
     return ckbatch::launch(plan->workspace, plan->grid, static_cast<size_t>(plan->lds_bytes),
-                           plan->sum_product != 0, one_lane_rows, static_cast<hipStream_t>(stream));
+                           plan->sum_product != 0, plan->kernel_variant,
+                           static_cast<hipStream_t>(stream));
 }
 
 namespace {

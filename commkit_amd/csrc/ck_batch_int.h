@@ -20,6 +20,7 @@ struct Shape {
     int F[kMaxClasses];    // frame slots per workgroup, per class
     int P[kMaxClasses];    // lanes per check row, per class
     int S[kMaxClasses];    // 1: the class's min-sum check states live in LDS
+    int one_lane;          // kernel variant: one-lane rows (0 none, 1 short, 2 long)
 };
 
 // Bytes of the descriptor at the start of a launch's workspace; the
@@ -34,16 +35,12 @@ int32_t shape(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
 void describe(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_method* method,
               const Shape& shape, void* workspace, void* host_desc);
 
-// Whether any of the first `num_classes` classes runs one lane per row
-// (those launches use the kernel compiled with that path).
-bool one_lane(const Shape& shape, int32_t num_classes);
-
 // Raises the kernels' dynamic LDS limit when needed (not under capture).
 void allow_lds(size_t lds_bytes);
 
 // Clears the queue counters and launches the decode on `stream`; the
 // descriptor must be in `workspace` by then (stream order).
-int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, bool one_lane_rows,
+int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, int one_lane,
                hipStream_t stream);
 
 }  // namespace ckbatch

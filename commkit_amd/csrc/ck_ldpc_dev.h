@@ -711,13 +711,13 @@ __device__ __forceinline__ void qc_sp_dispatch(int groups, const int2* tab, floa
 }
 
 // One sum-product check row `r` of block row `br`.
-template <int P, bool UPDATE_APP>
+template <int P, bool UPDATE_APP, int MAXT = kMaxDegree / P>
 __device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab, float* a,
                                           float* __restrict__ Rm, int br, int r, int p, bool first) {
     const int s0 = ld_uniform(c.qc_row_ptr, br);
     const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
     const int groups = ((dc + P - 1) / P + kSlotStep<P> - 1) / kSlotStep<P>;
-    qc_sp_dispatch<P, UPDATE_APP, kMaxDegree / P>(groups, tab, a, Rm, s0, dc, r, p, c.lifting,
+    qc_sp_dispatch<P, UPDATE_APP, MAXT>(groups, tab, a, Rm, s0, dc, r, p, c.lifting,
                                                   first);
 }
 

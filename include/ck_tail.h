@@ -215,6 +215,7 @@ typedef struct ck_batch_plan {
     int32_t sum_product;
     int32_t frames_per_group[8];
     int32_t lanes_per_row[8];
+    int32_t kernel_variant;  /* chosen by ck_batch_prepare, read by ck_batch_launch */
 } ck_batch_plan;
 
 /* Device workspace for a batch of up to 8 classes (-1 on bad arguments).
