@@ -288,10 +288,15 @@ __device__ __forceinline__ unsigned row_rank(int r) {
     return (static_cast<unsigned>(r) << 21) | 0x1fffffu;
 }
 
+// -4Z, the byte offset back over one block column, as a value the compiler
+// cannot fold into the offset arithmetic: the edge address then takes three
+// VALU ops (compare, select, three-way add).
+__device__ __forceinline__ int neg_col_bytes(int Z) { return __builtin_amdgcn_readfirstlane(-4 * Z); }
+
 // Byte offset from &a[r] of lane r's variable in the staged slot `v`
-// (rr = row_rank(r)).
-__device__ __forceinline__ int qc_off(int2 v, unsigned rr, int z4) {
-    return v.x + ((rr >= static_cast<unsigned>(v.y)) ? -z4 : 0);
+// (rr = row_rank(r), nz4 = neg_col_bytes(Z)).
+__device__ __forceinline__ int qc_off(int2 v, unsigned rr, int nz4) {
+    return v.x + ((rr >= static_cast<unsigned>(v.y)) ? nz4 : 0);
 }
 
 __device__ __forceinline__ float& lds_at(float* ar, int off) {
@@ -357,7 +362,7 @@ __device__ __forceinline__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2
         const int s1 = ld_uniform(c.qc_row_ptr, 1);
         float* ar = const_cast<float*>(app) + static_cast<size_t>(l.f) * N + l.r;
         const unsigned rr = row_rank(l.r);
-        const int z4 = 4 * Z;
+        const int z4 = neg_col_bytes(Z);
         unsigned parity = 0u;
         for (int s = s0; s < s1; s += U) {
             float v[U];
@@ -497,7 +502,7 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
     float q[NT];
     float* addr[NT];
     float* ar = a + r;
-    const int z4 = 4 * Z;
+    const int z4 = neg_col_bytes(Z);
     const unsigned rr = row_rank(r);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -649,7 +654,7 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
     float ph[NT];
     float* addr[NT];
     float* ar = a + r;
-    const int z4 = 4 * Z;
+    const int z4 = neg_col_bytes(Z);
     const unsigned rr = row_rank(r);
     float rold[NT];
 #pragma unroll
