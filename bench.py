@@ -39,7 +39,7 @@ MARGIN_DB = 0.3
 # (profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md).  Counters cannot be
 # read inside this process, so the figure is reported only for the workload
 # it was measured on.
-PROFILED_TRAFFIC = {"frames": FRAMES, "bytes": 2.280e9}
+PROFILED_TRAFFIC = {"frames": FRAMES, "bytes": 2.320e9}
 
 
 def _parse() -> argparse.Namespace:
