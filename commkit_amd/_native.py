@@ -52,6 +52,9 @@ SYMBOLS: dict[str, tuple[Any, list[Any]]] = {
     "ck_serve_release": (_i32, [_p, ctypes.c_uint64]),
     "ck_serve_usage": (_i32, [_p, _i32, _p, _p, _p]),
     "ck_serve_destroy": (_i32, [_p]),
+    "ck_ccdm_match": (_i32, [_p, _i64, _i64, _p, _i32, _i64, _p, _p]),
+    "ck_ccdm_dematch_workspace_bytes": (_i64, [_i64, _i64]),
+    "ck_ccdm_dematch": (_i32, [_p, _i64, _i64, _p, _i32, _i64, _i64, _p, _p, _p, _p]),
 }
 
 _LIB: ctypes.CDLL | None = None

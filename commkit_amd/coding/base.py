@@ -20,6 +20,7 @@ import numpy as np
 from .._array import as_2d, restore_1d, warn_if_double
 from ..backend import dispatch, to_host_like
 from . import ldpc as _ldpc
+from . import shaping as _shaping
 
 __all__ = ["DecodeResult", "decode", "encode"]
 
@@ -47,12 +48,30 @@ class DecodeResult:
     checked: bool
 
 
+_CODE_CLASSES = "LDPCCode, CCDM, PAS"
+
+
+def _is_code(code: Any) -> bool:
+    return isinstance(code, (_ldpc.LDPCCode, _shaping.CCDM, _shaping.PAS))
+
+
 def _resolve_method(code: Any, method: Any, function_name: str) -> Any:
-    if not isinstance(code, _ldpc.LDPCCode):
+    if not _is_code(code):
         raise TypeError(
-            f"{function_name}: code must be a code object (LDPCCode), "
+            f"{function_name}: code must be a code object ({_CODE_CLASSES}), "
             f"got {type(code).__name__}."
         )
+    if isinstance(code, _shaping.CCDM):
+        # The matcher's only decoder inverts the hard decisions
+        # (commkit/coding/shaping.py:223-246).
+        if method is not None:
+            raise TypeError(
+                f"{function_name}: method {type(method).__name__} does not decode "
+                "CCDM; its only decoder is the dematcher (method=None)."
+            )
+        return None
+    if isinstance(code, _shaping.PAS):
+        return _resolve_method(code.code, method, function_name)
     if method is None:
         return _ldpc.MinSum()
     if not isinstance(method, (_ldpc.MinSum, _ldpc.SumProduct)):
@@ -98,7 +117,18 @@ def decode(llrs: Any, *, code: Any, method: Any = None) -> DecodeResult:
     x2, was_1d = as_2d(x, name="llrs")
     blocks = _blocks(x2, code.n, "llrs", "decode()")
     logger.debug("Decoding %d block(s) of %r with %r.", blocks.shape[0], code, method)
-    bits, app, valid, iters = _ldpc._decode(code, blocks, method)
+    if isinstance(code, _shaping.CCDM):
+        # Hard decisions of the labels, then the dematcher; no soft output.
+        import torch
+
+        hard = (blocks < 0).to(torch.int8)
+        bits, valid = code._dematch_labels(hard)
+        app = None
+        iters = torch.ones(blocks.shape[0], dtype=torch.int32, device=blocks.device)
+    elif isinstance(code, _shaping.PAS):
+        bits, app, valid, iters = code._decode(blocks, method)
+    else:
+        bits, app, valid, iters = _ldpc._decode(code, blocks, method)
     num_blocks = x2.shape[-1] // code.n
     rows = x2.shape[0]
 
@@ -114,7 +144,7 @@ def decode(llrs: Any, *, code: Any, method: Any = None) -> DecodeResult:
     kind = placed.kind
     return DecodeResult(
         bits=to_host_like(shaped(bits, code.k), kind),
-        llrs=to_host_like(shaped(app, code.n), kind),
+        llrs=None if app is None else to_host_like(shaped(app, code.n), kind),
         valid=to_host_like(shaped(valid, 1), kind),
         iterations=to_host_like(shaped(iters, 1), kind),
         checked=True,
@@ -126,9 +156,10 @@ def encode(bits: Any, *, code: Any) -> Any:
     into int8 transmitted bits ``(..., N / k * n)`` on the input's device."""
     import torch
 
-    if not isinstance(code, _ldpc.LDPCCode):
+    if not _is_code(code):
         raise TypeError(
-            f"encode(): code must be a code object (LDPCCode), got {type(code).__name__}."
+            f"encode(): code must be a code object ({_CODE_CLASSES}), got "
+            f"{type(code).__name__}."
         )
     placed = dispatch(bits, "bits")
     x = placed.tensor
@@ -140,6 +171,8 @@ def encode(bits: Any, *, code: Any) -> Any:
     blocks = _blocks(x2, code.k, "bits", "encode()").to(torch.int8)
     if blocks.shape[0] == 0:
         out = torch.empty((0, code.n), dtype=torch.int8, device=x.device)
+    elif isinstance(code, (_shaping.CCDM, _shaping.PAS)):
+        out = code._encode(blocks)
     else:
         out = _ldpc._encode(code, blocks)
     num_blocks = x2.shape[-1] // code.k

@@ -324,6 +324,35 @@ int32_t ck_serve_usage(const ck_serve* server, int32_t code, int64_t* in_use,
 /* Waits for launches in flight, frees the engine's host resources. */
 int32_t ck_serve_destroy(ck_serve* server);
 
+/* ---------------------------------------------------------------------
+ * Constant-composition distribution matching (the amplitude matcher of
+ * probabilistic amplitude shaping).  `composition` is a HOST array of
+ * num_points (1..64) non-negative counts summing to the block length n
+ * (<= 2^30); the other arrays are device arrays.
+ * ------------------------------------------------------------------- */
+
+/* Matches each row of `bits` (rows, k) int8 0/1 to n point indices,
+ * `out` (rows, n) int32: the range coder of ccdm_match
+ * (commkit/coding/_numba_loops.py:944-1004; CCDM._match,
+ * commkit/coding/shaping.py:178-184). */
+int32_t ck_ccdm_match(const int8_t* bits, int64_t rows, int64_t k,
+                      const int64_t* composition, int32_t num_points, int64_t n,
+                      int32_t* out, void* stream);
+
+/* Device workspace of ck_ccdm_dematch: buffer_bits / 32 + 2 32-bit limbs a
+ * row (-1 on bad arguments). */
+int64_t ck_ccdm_dematch_workspace_bytes(int64_t rows, int64_t buffer_bits);
+
+/* Dematches each row of `symbols` (rows, n) int32 to `bits` (rows, k) int8
+ * and `valid` (rows,) uint8 (0: not a sequence the matcher emits, or its
+ * interval needs more than buffer_bits fractional bits): ccdm_dematch
+ * (commkit/coding/_numba_loops.py:1010-1102; CCDM._dematch,
+ * commkit/coding/shaping.py:186-198). */
+int32_t ck_ccdm_dematch(const int32_t* symbols, int64_t rows, int64_t n,
+                        const int64_t* composition, int32_t num_points, int64_t k,
+                        int64_t buffer_bits, int8_t* bits, uint8_t* valid,
+                        void* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,12 @@
  *   oracle_llr_points        commkit/mapping/_numba_llr.py:35-124
  *   oracle_llr_separable     commkit/mapping/_numba_llr.py:84-92,127-176
  *   oracle_llr_separable_maxlog  commkit/mapping/_numba_llr.py:179-235
+ *   oracle_ccdm_match        commkit/coding/_numba_loops.py:944-1004
+ *                            (ccdm_shares, ccdm_match): the constant-
+ *                            composition matcher's 64-bit range coder.
+ *   oracle_ccdm_dematch      commkit/coding/_numba_loops.py:1010-1102
+ *                            (low_bit, ccdm_dematch): its inverse, with the
+ *                            interval start kept exactly in 32-bit limbs.
  * The reference LLR kernels are fast-math (reassociation allowed), so LLRs
  * match them to rounding, not bit for bit.
  */
@@ -391,5 +397,145 @@ void oracle_llr_separable_maxlog(const float* x_re, const float* x_im, int64_t n
             axis_maxlog(x_im[i], lev_im, lab_im, lo_im, hi_im, pos_im, L, kb, inv_s2,
                         out + i * k);
         }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Constant-composition distribution matching                               */
+/* ------------------------------------------------------------------------ */
+
+#define CCDM_TOP ((int64_t)1 << 32)
+#define CCDM_HALF ((int64_t)1 << 31)
+#define CCDM_LIMB 0xFFFFFFFFull
+
+/* rng split in proportion to counts / remaining, floor rounding; the
+ * remainder goes to the largest count (lowest index among equals). */
+static void ccdm_split(const int64_t* counts, int M, int64_t rng, int64_t remaining,
+                       int64_t* shares) {
+    int big = 0;
+    int64_t rest = rng;
+    for (int i = 0; i < M; ++i) {
+        shares[i] = rng * counts[i] / remaining;
+        rest -= shares[i];
+        if (counts[i] > counts[big]) big = i;
+    }
+    shares[big] += rest;
+}
+
+/* bits (R, k) int8 -> out (R, n) point indices. */
+void oracle_ccdm_match(const int8_t* bits, int64_t R, int64_t k, const int64_t* composition,
+                       int32_t M, int64_t n, int32_t* out) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t b = 0; b < R; ++b) {
+        int64_t counts[256], shares[256];
+        memcpy(counts, composition, sizeof(int64_t) * (size_t)M);
+        const int8_t* u = bits + b * k;
+        int64_t pos = 0, d = 0, rng = CCDM_TOP, remaining = n;
+        for (int j = 0; j < 32; ++j, ++pos) d = (d << 1) | (pos < k ? u[pos] : 0);
+        for (int64_t t = 0; t < n; ++t) {
+            ccdm_split(counts, M, rng, remaining, shares);
+            int64_t cum = 0, share = 0;
+            int s = 0;
+            for (int i = 0; i < M; ++i) {
+                share = shares[i];
+                if (d < cum + share) {
+                    s = i;
+                    break;
+                }
+                cum += share;
+            }
+            out[b * n + t] = s;
+            d -= cum;
+            rng = share;
+            --counts[s];
+            --remaining;
+            while (rng < CCDM_HALF) {
+                rng <<= 1;
+                d = (d << 1) | (pos < k ? u[pos] : 0);
+                ++pos;
+            }
+        }
+    }
+}
+
+/* Bit 2^-j (j >= 1) of the limbs (32 bits each, most significant first). */
+static int ccdm_bit(const uint64_t* low, int64_t j) {
+    return (int)((low[(j - 1) >> 5] >> (31 - ((j - 1) & 31))) & 1u);
+}
+
+/* symbols (R, n) point indices -> bits (R, k), valid (R,). */
+void oracle_ccdm_dematch(const int32_t* symbols, int64_t R, int64_t n,
+                         const int64_t* composition, int32_t M, int64_t k,
+                         int64_t buffer_bits, int8_t* bits, uint8_t* valid) {
+    const int64_t limbs = buffer_bits / 32 + 2;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t b = 0; b < R; ++b) {
+        int64_t counts[256], shares[256];
+        memcpy(counts, composition, sizeof(int64_t) * (size_t)M);
+        uint64_t* low = (uint64_t*)calloc((size_t)limbs, sizeof(uint64_t));
+        int64_t rng = CCDM_TOP, shift = 32, remaining = n;
+        int ok = 1;
+        for (int64_t t = 0; t < n; ++t) {
+            const int32_t s = symbols[b * n + t];
+            if (s < 0 || s >= M || counts[s] == 0) {
+                ok = 0;
+                break;
+            }
+            ccdm_split(counts, M, rng, remaining, shares);
+            int64_t cum = 0;
+            for (int i = 0; i < s; ++i) cum += shares[i];
+            /* low += cum 2^-shift, the carry running towards 2^-1. */
+            int64_t i = (shift - 1) >> 5;
+            const uint64_t v = (uint64_t)cum << (31 - ((shift - 1) & 31));
+            low[i] += v & CCDM_LIMB;
+            uint64_t carry = (low[i] >> 32) + (v >> 32);
+            low[i] &= CCDM_LIMB;
+            while (carry && i > 0) {
+                --i;
+                low[i] += carry;
+                carry = low[i] >> 32;
+                low[i] &= CCDM_LIMB;
+            }
+            rng = shares[s];
+            --counts[s];
+            --remaining;
+            while (rng < CCDM_HALF) {
+                rng <<= 1;
+                ++shift;
+            }
+            if (shift > buffer_bits) {
+                ok = 0;
+                break;
+            }
+        }
+        /* v = ceil(low 2^k): the first k bits, plus one if any bit follows. */
+        int8_t* out = bits + b * k;
+        int beyond = 0;
+        const int64_t first = k >> 5;
+        if (low[first] & (CCDM_LIMB >> (k & 31))) beyond = 1;
+        for (int64_t l = first + 1; l < limbs && !beyond; ++l)
+            if (low[l]) beyond = 1;
+        for (int64_t j = 1; j <= k; ++j) out[j - 1] = (int8_t)ccdm_bit(low, j);
+        int up = beyond;
+        for (int64_t j = k; up && j > 0; --j) {
+            up = out[j - 1];
+            out[j - 1] ^= 1;
+        }
+        if (ok && up) ok = 0; /* v = 2^k: no payload */
+        if (ok && beyond) {
+            /* v 2^-k must lie in [low, low + rng 2^-shift): every bit of low
+             * after the k-th above its last 32 is one, and 2^w - (those 32
+             * bits) < rng. */
+            const int64_t lo = (k + 1 > shift - 31) ? k + 1 : shift - 31;
+            for (int64_t j = k + 1; j < lo && ok; ++j)
+                if (!ccdm_bit(low, j)) ok = 0;
+            if (ok) {
+                int64_t f = 0;
+                for (int64_t j = lo; j <= shift; ++j) f = (f << 1) | ccdm_bit(low, j);
+                ok = ((int64_t)1 << (shift - lo + 1)) - f < rng;
+            }
+        }
+        valid[b] = (uint8_t)ok;
+        free(low);
     }
 }

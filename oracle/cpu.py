@@ -29,6 +29,8 @@ _SIGS = {
                              _i32, _p],
     "oracle_llr_separable_maxlog": [_p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32,
                                     _i32, _f32, _p],
+    "oracle_ccdm_match": [_p, _i64, _i64, _p, _i32, _i64, _p],
+    "oracle_ccdm_dematch": [_p, _i64, _i64, _p, _i32, _i64, _i64, _p, _p],
 }  # fmt: skip
 
 _LIB: ctypes.CDLL | None = None
@@ -143,3 +145,31 @@ def llr_separable_maxlog(x: np.ndarray, axes, inv_s2: float) -> np.ndarray:
     lib().oracle_llr_separable_maxlog(_ptr(xr), _ptr(xi), xr.size, *[_ptr(a) for a in t], L, kb,
                                       float(np.float32(inv_s2)), _ptr(out))  # fmt: skip
     return out.reshape(-1)
+
+
+def ccdm_match(bits: np.ndarray, composition: np.ndarray) -> np.ndarray:
+    """``oracle_ccdm_match``: (R, k) bits to (R, n) point indices, n = sum of the
+    composition."""
+    bits = np.ascontiguousarray(bits, dtype=np.int8)
+    comp = np.ascontiguousarray(composition, dtype=np.int64)
+    if comp.size > 256:
+        raise ValueError("the oracle's matcher takes at most 256 points")
+    n = int(comp.sum())
+    out = np.empty((bits.shape[0], n), dtype=np.int32)
+    lib().oracle_ccdm_match(_ptr(bits), bits.shape[0], bits.shape[1], _ptr(comp), comp.size, n,
+                            _ptr(out))  # fmt: skip
+    return out
+
+
+def ccdm_dematch(symbols: np.ndarray, composition: np.ndarray, k: int,
+                 buffer_bits: int) -> tuple[np.ndarray, np.ndarray]:
+    """``oracle_ccdm_dematch``: (R, n) point indices to (R, k) bits and (R,) validity."""
+    sym = np.ascontiguousarray(symbols, dtype=np.int32)
+    comp = np.ascontiguousarray(composition, dtype=np.int64)
+    if comp.size > 256:
+        raise ValueError("the oracle's dematcher takes at most 256 points")
+    bits = np.empty((sym.shape[0], k), dtype=np.int8)
+    valid = np.empty(sym.shape[0], dtype=np.uint8)
+    lib().oracle_ccdm_dematch(_ptr(sym), sym.shape[0], sym.shape[1], _ptr(comp), comp.size, k,
+                              buffer_bits, _ptr(bits), _ptr(valid))  # fmt: skip
+    return bits, valid.astype(bool)
