@@ -14,11 +14,13 @@
 // A block is a sequential recursion, so one lane decodes one block and the
 // launch is parallel over blocks.  The composition (at most kMaxPoints
 // points) comes in the kernel arguments; each lane keeps its counts and the
-// current split in LDS, point-major so that a wave's lanes touch
-// consecutive words.  floor(a / remaining) with a < 2^62 is a float64
-// estimate from one reciprocal per point position, corrected exactly with
-// 64-bit integer arithmetic.  The dematcher's interval start lives in a
-// caller workspace of buffer_bits / 32 + 2 limbs per block, limb-major.
+// current split in registers (up to 16 points; larger alphabets compile to
+// a variant whose arrays the compiler keeps in scratch).  floor(a /
+// remaining) with a < 2^62 is a float64 estimate from one reciprocal per
+// position, corrected exactly with 64-bit integer arithmetic.  The
+// dematcher accumulates the interval start in a 128-bit register window
+// and writes the limbs it leaves behind to a caller workspace of
+// buffer_bits / 32 + 2 limbs per block, limb-major.
 #include "ck_common.h"
 #include "ck_tail.h"
 
@@ -51,35 +53,93 @@ __device__ __forceinline__ int64_t div_floor(int64_t a, int64_t d, double inv) {
     return q;
 }
 
-// The split of `rng` at this position into share[i * kLanes + lane].
-__device__ __forceinline__ void split(const int* cnt, int64_t* share, int M, int lane,
-                                      int64_t rng, int64_t remaining) {
-    const double inv = 1.0 / static_cast<double>(remaining);
-    int big = 0;
-    int cbig = cnt[lane];
-    int64_t rest = rng;
-    for (int i = 0; i < M; ++i) {
-        const int c = cnt[i * kLanes + lane];
-        const int64_t s = div_floor(rng * c, remaining, inv);
-        share[i * kLanes + lane] = s;
-        rest -= s;
-        if (c > cbig) {
-            big = i;
-            cbig = c;
-        }
-    }
-    share[big * kLanes + lane] += rest;
-}
+// A lane's remaining counts and the current split, MB >= M entries (the
+// unused ones count zero, get no share and are never chosen).  Loops over
+// MB unroll, so with MB <= 16 both arrays live in registers.
+template <int MB>
+struct Coder {
+    int cnt[MB];
+    int64_t share[MB];
 
+    __device__ __forceinline__ void init(const Composition& comp, int M) {
+#pragma unroll
+        for (int i = 0; i < MB; ++i) cnt[i] = i < M ? static_cast<int>(comp.count[i]) : 0;
+    }
+
+    // floor(rng * c_i / remaining) each, the remainder to the largest count
+    // (the lowest index among equals).
+    __device__ __forceinline__ void split(int64_t rng, int64_t remaining) {
+        const double inv = 1.0 / static_cast<double>(remaining);
+        int big = 0;
+        int cbig = cnt[0];
+        int64_t rest = rng;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            share[i] = div_floor(rng * cnt[i], remaining, inv);
+            rest -= share[i];
+            const bool more = cnt[i] > cbig;
+            big = more ? i : big;
+            cbig = more ? cnt[i] : cbig;
+        }
+#pragma unroll
+        for (int i = 0; i < MB; ++i) share[i] += (i == big) ? rest : 0;
+    }
+
+    // The point whose interval holds d: the first s with d < cum_s + share_s;
+    // its cumulative start and share.
+    __device__ __forceinline__ int find(int64_t d, int64_t* cum_out, int64_t* share_out) const {
+        int64_t cum = 0;
+        int s = -1;
+        int64_t cs = 0;
+        int64_t ss = 0;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            const bool hit = s < 0 && d < cum + share[i];
+            s = hit ? i : s;
+            cs = hit ? cum : cs;
+            ss = hit ? share[i] : ss;
+            cum += share[i];
+        }
+        *cum_out = cs;
+        *share_out = ss;
+        return s < 0 ? 0 : s;
+    }
+
+    __device__ __forceinline__ int64_t start_of(int s) const {
+        int64_t cum = 0;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) cum += (i < s) ? share[i] : 0;
+        return cum;
+    }
+
+    __device__ __forceinline__ int64_t share_of(int s) const {
+        int64_t v = 0;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) v = (i == s) ? share[i] : v;
+        return v;
+    }
+
+    __device__ __forceinline__ int count_of(int s) const {
+        int v = 0;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) v = (i == s) ? cnt[i] : v;
+        return v;
+    }
+
+    __device__ __forceinline__ void take(int s) {
+#pragma unroll
+        for (int i = 0; i < MB; ++i) cnt[i] -= (i == s) ? 1 : 0;
+    }
+};
+
+template <int MB>
 __global__ __launch_bounds__(kLanes) void ccdm_match_kernel(const signed char* __restrict__ bits,
                                                             long long R, long long k, Composition comp,
                                                             int M, long long n, int* __restrict__ out) {
-    __shared__ int cnt[kMaxPoints * kLanes];
-    __shared__ int64_t share[kMaxPoints * kLanes];
-    const int lane = static_cast<int>(threadIdx.x);
-    const long long b = static_cast<long long>(blockIdx.x) * kLanes + lane;
+    const long long b = static_cast<long long>(blockIdx.x) * kLanes + threadIdx.x;
     if (b >= R) return;  // no barriers below: each lane works alone
-    for (int i = 0; i < M; ++i) cnt[i * kLanes + lane] = static_cast<int>(comp.count[i]);
+    Coder<MB> cd;
+    cd.init(comp, M);
     const signed char* u = bits + b * k;
     int* o = out + b * n;
     long long pos = 0;
@@ -88,22 +148,14 @@ __global__ __launch_bounds__(kLanes) void ccdm_match_kernel(const signed char* _
     int64_t rng = kTop;
     int64_t remaining = n;
     for (long long t = 0; t < n; ++t) {
-        split(cnt, share, M, lane, rng, remaining);
-        int64_t cum = 0;
-        int64_t sh = 0;
-        int s = 0;
-        for (int i = 0; i < M; ++i) {
-            sh = share[i * kLanes + lane];
-            if (d < cum + sh) {
-                s = i;
-                break;
-            }
-            cum += sh;
-        }
+        cd.split(rng, remaining);
+        int64_t cum;
+        int64_t sh;
+        const int s = cd.find(d, &cum, &sh);
         o[t] = s;
         d -= cum;
         rng = sh;
-        cnt[s * kLanes + lane] -= 1;
+        cd.take(s);
         --remaining;
         while (rng < kHalf) {
             rng <<= 1;
@@ -118,16 +170,27 @@ __device__ __forceinline__ int limb_bit(const unsigned* low, long long R, long l
     return static_cast<int>((low[((j - 1) >> 5) * R] >> (31 - ((j - 1) & 31))) & 1u);
 }
 
+// The start of the interval, low = sum over positions of cum 2^-shift, is
+// accumulated in a 128-bit register window over limbs base - 2 .. base + 1
+// (hi: the first two, lo: the last two; limbs below zero are not stored).
+// Each addition lands on limbs (i - 1, i) with i = (shift - 1) / 32, which
+// only grows: the window slides down to base = i - 1 first, writing out its
+// first limb, which no later addition reaches except through a carry out
+// of all 128 bits (two limbs of ones), taken through memory.
+__device__ __forceinline__ void emit(unsigned* low, long long R, long long limbs, long long l,
+                                     uint64_t word) {
+    if (l >= 0 && l < limbs) low[l * R] = static_cast<unsigned>(word);
+}
+
+template <int MB>
 __global__ __launch_bounds__(kLanes) void ccdm_dematch_kernel(
     const int* __restrict__ symbols, long long R, long long n, Composition comp, int M,
     long long k, long long buffer_bits, signed char* __restrict__ bits,
     unsigned char* __restrict__ valid, unsigned* __restrict__ ws) {
-    __shared__ int cnt[kMaxPoints * kLanes];
-    __shared__ int64_t share[kMaxPoints * kLanes];
-    const int lane = static_cast<int>(threadIdx.x);
-    const long long b = static_cast<long long>(blockIdx.x) * kLanes + lane;
+    const long long b = static_cast<long long>(blockIdx.x) * kLanes + threadIdx.x;
     if (b >= R) return;
-    for (int i = 0; i < M; ++i) cnt[i * kLanes + lane] = static_cast<int>(comp.count[i]);
+    Coder<MB> cd;
+    cd.init(comp, M);
     const long long limbs = buffer_bits / 32 + 2;
     unsigned* low = ws + b;  // limb l at low[l * R]
     for (long long l = 0; l < limbs; ++l) low[l * R] = 0u;
@@ -136,29 +199,39 @@ __global__ __launch_bounds__(kLanes) void ccdm_dematch_kernel(
     long long shift = 32;
     int64_t remaining = n;
     bool ok = true;
+    uint64_t hi = 0;  // limbs base - 2, base - 1
+    uint64_t lo = 0;  // limbs base, base + 1
+    long long base = -1;
     for (long long t = 0; t < n; ++t) {
         const int s = sym[t];
-        if (s < 0 || s >= M || cnt[s * kLanes + lane] == 0) {
+        if (s < 0 || s >= M || cd.count_of(s) == 0) {
             ok = false;
             break;
         }
-        split(cnt, share, M, lane, rng, remaining);
-        int64_t cum = 0;
-        for (int i = 0; i < s; ++i) cum += share[i * kLanes + lane];
-        // low += cum 2^-shift, the carry running towards 2^-1.
-        long long i = (shift - 1) >> 5;
-        const uint64_t v = static_cast<uint64_t>(cum) << (31 - ((shift - 1) & 31));
-        uint64_t acc = static_cast<uint64_t>(low[i * R]) + (v & 0xFFFFFFFFull);
-        low[i * R] = static_cast<unsigned>(acc);
-        uint64_t carry = (acc >> 32) + (v >> 32);
-        while (carry != 0 && i > 0) {
-            --i;
-            acc = static_cast<uint64_t>(low[i * R]) + carry;
-            low[i * R] = static_cast<unsigned>(acc);
-            carry = acc >> 32;
+        cd.split(rng, remaining);
+        const int64_t cum = cd.start_of(s);
+        const long long i = (shift - 1) >> 5;
+        while (base < i - 1) {
+            emit(low, R, limbs, base - 2, hi >> 32);
+            hi = (hi << 32) | (lo >> 32);
+            lo <<= 32;
+            ++base;
         }
-        rng = share[s * kLanes + lane];
-        cnt[s * kLanes + lane] -= 1;
+        // low += cum 2^-shift: v on limbs (i - 1, i) = the window's lo.
+        const uint64_t v = static_cast<uint64_t>(cum) << (31 - ((shift - 1) & 31));
+        lo += v;
+        if (lo < v) {
+            hi += 1;
+            if (hi == 0) {  // through limbs base - 3, base - 4, ... (none above 2^-1)
+                for (long long l = base - 3; l >= 0; --l) {
+                    const uint64_t x = static_cast<uint64_t>(low[l * R]) + 1u;
+                    low[l * R] = static_cast<unsigned>(x);
+                    if ((x >> 32) == 0) break;
+                }
+            }
+        }
+        rng = cd.share_of(s);
+        cd.take(s);
         --remaining;
         while (rng < kHalf) {
             rng <<= 1;
@@ -169,6 +242,10 @@ __global__ __launch_bounds__(kLanes) void ccdm_dematch_kernel(
             break;
         }
     }
+    emit(low, R, limbs, base - 2, hi >> 32);
+    emit(low, R, limbs, base - 1, hi);
+    emit(low, R, limbs, base, lo >> 32);
+    emit(low, R, limbs, base + 1, lo);
     // v = ceil(low 2^k): the first k bits, plus one if any bit follows.
     signed char* out = bits + b * k;
     const long long first = k >> 5;
@@ -184,13 +261,13 @@ __global__ __launch_bounds__(kLanes) void ccdm_dematch_kernel(
     if (ok && beyond) {
         // v 2^-k must lie in [low, low + rng 2^-shift): every bit of low after
         // the k-th above its last 32 is one, and 2^w - (those bits) < rng.
-        const long long lo = (k + 1 > shift - 31) ? k + 1 : shift - 31;
-        for (long long j = k + 1; j < lo && ok; ++j)
+        const long long lo_bit = (k + 1 > shift - 31) ? k + 1 : shift - 31;
+        for (long long j = k + 1; j < lo_bit && ok; ++j)
             if (!limb_bit(low, R, j)) ok = false;
         if (ok) {
             int64_t f = 0;
-            for (long long j = lo; j <= shift; ++j) f = (f << 1) | limb_bit(low, R, j);
-            ok = (int64_t{1} << (shift - lo + 1)) - f < rng;
+            for (long long j = lo_bit; j <= shift; ++j) f = (f << 1) | limb_bit(low, R, j);
+            ok = (int64_t{1} << (shift - lo_bit + 1)) - f < rng;
         }
     }
     valid[b] = ok ? 1 : 0;
@@ -230,10 +307,19 @@ extern "C" int32_t ck_ccdm_match(const int8_t* bits, int64_t rows, int64_t k,
     if (rows == 0) return CK_OK;
     CK_REQUIRE(rows <= (int64_t{1} << 31) * kLanes, "too many rows");
     CK_REQUIRE(bits != nullptr && out != nullptr, "NULL bits or output");
-    hipLaunchKernelGGL(ccdm_match_kernel, dim3(blocks_of(rows)), dim3(kLanes), 0,
-                       static_cast<hipStream_t>(stream), reinterpret_cast<const signed char*>(bits),
-                       static_cast<long long>(rows), static_cast<long long>(k), comp, num_points,
-                       static_cast<long long>(n), out);
+    const dim3 g(blocks_of(rows)), blk(kLanes);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const signed char* u = reinterpret_cast<const signed char*>(bits);
+    const long long R = rows, kk = k, nn = n;
+    if (num_points <= 4)
+        hipLaunchKernelGGL(ccdm_match_kernel<4>, g, blk, 0, st, u, R, kk, comp, num_points, nn, out);
+    else if (num_points <= 8)
+        hipLaunchKernelGGL(ccdm_match_kernel<8>, g, blk, 0, st, u, R, kk, comp, num_points, nn, out);
+    else if (num_points <= 16)
+        hipLaunchKernelGGL(ccdm_match_kernel<16>, g, blk, 0, st, u, R, kk, comp, num_points, nn, out);
+    else
+        hipLaunchKernelGGL(ccdm_match_kernel<kMaxPoints>, g, blk, 0, st, u, R, kk, comp, num_points, nn,
+                           out);
     return ck::launched("ccdm match");
 }
 
@@ -256,11 +342,22 @@ extern "C" int32_t ck_ccdm_dematch(const int32_t* symbols, int64_t rows, int64_t
     CK_REQUIRE(rows <= (int64_t{1} << 31) * kLanes, "too many rows");
     CK_REQUIRE(symbols != nullptr && bits != nullptr && valid != nullptr && workspace != nullptr,
                "NULL symbols, bits, valid or workspace");
-    hipLaunchKernelGGL(ccdm_dematch_kernel, dim3(blocks_of(rows)), dim3(kLanes), 0,
-                       static_cast<hipStream_t>(stream), symbols, static_cast<long long>(rows),
-                       static_cast<long long>(n), comp, num_points, static_cast<long long>(k),
-                       static_cast<long long>(buffer_bits),
-                       reinterpret_cast<signed char*>(bits), valid,
-                       static_cast<unsigned*>(workspace));
+    const dim3 g(blocks_of(rows)), blk(kLanes);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    signed char* out = reinterpret_cast<signed char*>(bits);
+    unsigned* ws = static_cast<unsigned*>(workspace);
+    const long long R = rows, nn = n, kk = k, bb = buffer_bits;
+    if (num_points <= 4)
+        hipLaunchKernelGGL(ccdm_dematch_kernel<4>, g, blk, 0, st, symbols, R, nn, comp, num_points,
+                           kk, bb, out, valid, ws);
+    else if (num_points <= 8)
+        hipLaunchKernelGGL(ccdm_dematch_kernel<8>, g, blk, 0, st, symbols, R, nn, comp, num_points,
+                           kk, bb, out, valid, ws);
+    else if (num_points <= 16)
+        hipLaunchKernelGGL(ccdm_dematch_kernel<16>, g, blk, 0, st, symbols, R, nn, comp, num_points,
+                           kk, bb, out, valid, ws);
+    else
+        hipLaunchKernelGGL(ccdm_dematch_kernel<kMaxPoints>, g, blk, 0, st, symbols, R, nn, comp,
+                           num_points, kk, bb, out, valid, ws);
     return ck::launched("ccdm dematch");
 }

@@ -88,24 +88,27 @@ def frame_error_rate(code: Any, constellation: Any, esn0_db: float, frames: int,
 
 def shaped_frame_error_rate(code: Any, constellation: Any, esn0_db: float, frames: int,
                             seed: int, method: Any = None) -> float:
-    """FER of ``frames`` frames of the shaping stand-in at ``esn0_db`` (GPU)."""
+    """FER of ``frames`` codewords of probabilistic amplitude shaping at
+    ``esn0_db`` (GPU): exact LLRs with the prior, the code's decoder, the
+    dematcher; a frame fails when its payload differs or is flagged."""
     import torch
 
-    from .coding.ldpc import SumProduct, _decode
+    from .coding.ldpc import SumProduct
     from .mapping.demap import llr_any
     from .workloads import make_pas_frames
 
     method = method or SumProduct(max_iterations=30)
     fr = make_pas_frames(code, constellation, esn0_db, frames, seed=seed)
     llr = llr_any(constellation, fr.rx, float(fr.noise_var[0]), "exact").reshape(frames, code.n)
-    bits, _, _, _ = _decode(code, llr[:, fr.iperm].contiguous(), method)
-    wrong = (bits != fr.info).any(dim=1)
+    bits, _, valid, _ = fr.pas._decode(llr, method)
+    wrong = (bits != fr.info).any(dim=1) | ~valid
     torch.cuda.synchronize()
     return float(wrong.float().mean())
 
 
 def calibrate_shaped(code_name: str, order: int, nu: float, *, verbose: bool = True) -> float:
-    """Measure the waterfall of the shaping stand-in (SumProduct, 30 iterations)."""
+    """Measure the waterfall of probabilistic amplitude shaping over a code
+    (SumProduct, 30 iterations)."""
     from .mapping.constellation import Constellation
     from .workloads import seeded_code
 

@@ -34,7 +34,7 @@ from .mapping.constellation import Constellation
 
 __all__ = ["CODES", "seeded_base", "seeded_code", "Frames", "make_frames", "code_by_name",
            "MIXED_CODES", "MIXED_ORDERS", "Mixed", "MixedGroup", "make_mixed", "PasFrames",
-           "pas_layout", "make_pas_frames"]
+           "make_pas_frames"]
 
 
 @dataclass(frozen=True)
@@ -249,17 +249,19 @@ def make_mixed(num_frames: int = 10_000, *, seed: int = 0, device: Any = None,
 
 @dataclass
 class PasFrames:
-    """Frames of the probabilistic-amplitude-shaping stand-in.
+    """Codewords of probabilistic amplitude shaping.
 
-    info (F, k) int8 and code_bits (F, n) int8 in codeword order; ``perm``
-    (n,) int64 device: transmitted bit ``t`` carries codeword bit
+    ``pas`` the :class:`~commkit_amd.coding.PAS` code; info (F, pas.k) int8
+    uniform payload bits; sent (F, n) int8 transmitted (label) bits;
+    ``perm`` (n,) int64 device: transmitted bit ``t`` is codeword bit
     ``perm[t]``, and ``iperm`` its inverse (codeword LLRs are the received
     bit LLRs gathered by ``iperm``); tx, rx (F, n / k_s) complex64;
     esn0_db, noise_var (F,) float64 host.
     """
 
+    pas: Any
     info: Any
-    code_bits: Any
+    sent: Any
     perm: Any
     iperm: Any
     tx: Any
@@ -268,95 +270,24 @@ class PasFrames:
     noise_var: np.ndarray
 
 
-def _sign_bit(axis: Any) -> int:
-    """Index (within the axis label) of the bit that is the level's sign."""
-    lab = np.asarray(axis.labels)
-    half = lab.shape[0] // 2
-    for j in range(lab.shape[1]):
-        lo, hi = lab[:half, j], lab[half:, j]
-        if (lo == lo[0]).all() and (hi == hi[0]).all() and lo[0] != hi[0]:
-            return j
-    raise ValueError("no sign bit on this axis")
-
-
-def pas_layout(code: LDPCCode, constellation: Constellation) -> np.ndarray:
-    """Bit layout of the shaping stand-in: ``perm[t]`` is the codeword bit
-    sent as transmitted bit ``t`` (symbol ``t // k_s``, label bit
-    ``t % k_s``).  Amplitude bits, then the first sign bits, carry the
-    information positions ``0 .. k-1`` in order; the last ``m`` sign bits
-    carry the parity, as probabilistic amplitude shaping puts the uniform
-    parity bits on signs."""
-    axes = constellation._axes
-    if axes is None:
-        raise ValueError("shaping stand-in needs a separable constellation")
-    ks = constellation.bits_per_symbol
-    n, k = code.n, code.k
-    if n % ks:
-        raise ValueError(f"n = {n} is not a multiple of {ks}")
-    S = n // ks
-    sign_pos = sorted(int(ax.bits[_sign_bit(ax)]) for ax in axes)
-    amp_pos = [b for b in range(ks) if b not in sign_pos]
-    t_amp = (np.arange(S)[:, None] * ks + np.asarray(amp_pos)[None, :]).reshape(-1)
-    t_sign = (np.arange(S)[:, None] * ks + np.asarray(sign_pos)[None, :]).reshape(-1)
-    m = n - k
-    if m > t_sign.size:
-        raise ValueError(f"{m} parity bits do not fit on {t_sign.size} sign bits")
-    perm = np.empty(n, dtype=np.int64)
-    info_slots = np.concatenate([t_amp, t_sign[: t_sign.size - m]])
-    perm[info_slots] = np.arange(k)
-    perm[t_sign[t_sign.size - m:]] = np.arange(k, n)
-    return perm
-
-
 def make_pas_frames(code: LDPCCode, constellation: Constellation, esn0_db: Any,
                     num_frames: int, *, seed: int = 0, device: Any = None) -> PasFrames:
-    """Codewords of a shaped constellation through an i.i.d.-amplitude
-    stand-in of probabilistic amplitude shaping (the reference's PAS,
-    ``commkit/coding/shaping.py:69-433``, matches amplitudes with a CCDM):
-    each axis amplitude is drawn from the constellation's marginal prior,
-    its label bits are information bits, the remaining information bits
-    ride on signs, and the code's parity goes on the last sign bits
-    (``pas_layout``).  AWGN as :func:`make_frames`."""
+    """Codewords of probabilistic amplitude shaping over ``code``: uniform
+    payload bits through the CCDM amplitude matcher, the code and the
+    sign/amplitude bit layout (``commkit/coding/shaping.py:391-397``, on the
+    GPU), Gray mapping and AWGN as :func:`make_frames`."""
     import torch
 
-    from .coding.ldpc import _encode
+    from .coding.shaping import PAS
 
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else device
-    ks = constellation.bits_per_symbol
-    n, k = code.n, code.k
-    S = n // ks
-    perm = pas_layout(code, constellation)
+    pas = PAS(code, constellation)
     rng = np.random.Generator(np.random.PCG64(seed))
-    b = np.zeros((num_frames, n), dtype=np.int8)
-    for ax in constellation._axes:
-        lab = np.asarray(ax.labels).astype(np.int8)
-        L = lab.shape[0]
-        half = L // 2
-        pmf = np.full(L, 1.0 / L) if ax.pmf is None else np.asarray(ax.pmf, dtype=np.float64)
-        amp_p = pmf[half:] + pmf[:half][::-1]
-        amp_p = amp_p / amp_p.sum()
-        j_sign = _sign_bit(ax)
-        # Mirror levels differ only in the sign bit (reflected Gray labels).
-        assert (np.delete(lab[half:], j_sign, axis=1) ==
-                np.delete(lab[:half][::-1], j_sign, axis=1)).all()  # fmt: skip
-        a = rng.choice(half, size=(num_frames, S), p=amp_p)
-        bits = lab[half + a]  # (F, S, k_axis): positive-half labels
-        for jj, pos in enumerate(np.asarray(ax.bits)):
-            if jj != j_sign:
-                b[:, pos::ks] = bits[:, :, jj]
-    info_slots = np.flatnonzero(perm < k)
-    sign_info = perm[info_slots] >= (S * (ks - 2))  # info positions carried by signs
-    b_info = b[:, info_slots]
-    b_info[:, sign_info] = rng.integers(0, 2, size=(num_frames, int(sign_info.sum())),
-                                        dtype=np.int8)  # fmt: skip
-    u = np.empty((num_frames, k), dtype=np.int8)
-    u[:, perm[info_slots]] = b_info
-    info = torch.from_numpy(u).to(dev)
-    words = torch.empty((num_frames, n), dtype=torch.int8, device=dev)
+    info = torch.from_numpy(rng.integers(0, 2, size=(num_frames, pas.k), dtype=np.int8)).to(dev)
+    sent = torch.empty((num_frames, code.n), dtype=torch.int8, device=dev)
     for r0 in range(0, num_frames, 1024):
-        words[r0 : r0 + 1024] = _encode(code, info[r0 : r0 + 1024])
-    perm_t = torch.from_numpy(perm).to(dev)
-    sent = words[:, perm_t]
+        sent[r0 : r0 + 1024] = pas._encode(info[r0 : r0 + 1024])
+    perm, iperm = pas._tables(dev)
     tx = constellation.map(sent)
     es = np.broadcast_to(np.asarray(esn0_db, dtype=np.float64), (num_frames,)).copy()
     gen = torch.Generator(device=dev)
@@ -370,4 +301,4 @@ def make_pas_frames(code: LDPCCode, constellation: Constellation, esn0_db: Any,
     )
     rx = (tx + noise * std).to(torch.complex64)
     nv = constellation.power() * 10.0 ** (-es / 10.0)
-    return PasFrames(info, words, perm_t, torch.argsort(perm_t), tx, rx, es, nv)
+    return PasFrames(pas, info, sent, perm, iperm, tx, rx, es, nv)

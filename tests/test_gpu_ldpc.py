@@ -205,9 +205,10 @@ def test_decode_contract(gpu):
 
 
 def test_shaped_pas_frames_decode(gpu):
-    """The shaping stand-in end to end at a high Es/N0: exact LLRs with the
-    prior, gathered into codeword order, decode by layered sum-product to
-    the information bits; the amplitudes follow the prior."""
+    """Probabilistic amplitude shaping frames end to end at a high Es/N0:
+    exact LLRs with the prior, gathered into codeword order, decode by
+    layered sum-product to the codeword's information part (the matcher's
+    amplitude bits, then the payload's sign bits)."""
     import torch
 
     from commkit_amd import Constellation, SumProduct
@@ -222,5 +223,7 @@ def test_shaped_pas_frames_decode(gpu):
     llr_c = llr[:, fr.iperm].contiguous()
     bits, app, valid, iters = gpu_decode(code, llr_c.cpu().numpy(), SumProduct(max_iterations=30))
     assert valid.all()
-    np.testing.assert_array_equal(bits, fr.info.cpu().numpy())
-    assert torch.equal(fr.code_bits[:, : code.k], fr.info)  # systematic
+    words = fr.sent[:, fr.iperm]  # codeword order
+    np.testing.assert_array_equal(bits, words[:, : code.k].cpu().numpy())  # systematic
+    ccdm = fr.pas.matcher
+    assert torch.equal(words[:, ccdm.n : code.k], fr.info[:, ccdm.k :])  # payload on signs
