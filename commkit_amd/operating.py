@@ -45,12 +45,12 @@ WATERFALL_DB: dict[tuple[str, int], float] = {
     ("r5/6-z360", 256): 22.65,
 }
 
-#: Measured waterfalls (dB) of the shaping stand-in
+#: Measured waterfalls (dB) of probabilistic amplitude shaping over a code
 #: (:func:`commkit_amd.workloads.make_pas_frames`) by (code name, QAM order,
 #: Maxwell-Boltzmann nu): exact LLRs with the prior, layered sum-product of
-#: at most 30 iterations, otherwise the definition above.
+#: at most 30 iterations and the dematcher, otherwise the definition above.
 SHAPED_WATERFALL_DB: dict[tuple[str, int, float], float] = {
-    ("r5/6-z360", 256, 0.05): 13.3,
+    ("r5/6-z360", 256, 0.05): 13.2,
 }
 
 _GRID = 0.05
@@ -168,7 +168,7 @@ if __name__ == "__main__":
     if len(sys.argv) > 1:
         pairs = [tuple(a.split(":")) for a in sys.argv[1:]]  # type: ignore[misc]
     for p in pairs:
-        if len(p) == 3:  # name:order:nu, the shaping stand-in
+        if len(p) == 3:  # name:order:nu, probabilistic amplitude shaping
             wf = calibrate_shaped(p[0], int(p[1]), float(p[2]))
             print(f"shaped waterfall {p[0]} {p[1]}-QAM nu={p[2]}: {wf} dB", flush=True)
         else:
