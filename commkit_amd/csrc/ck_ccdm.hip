@@ -11,8 +11,10 @@
 // (32-bit limbs) and rounds it up to k bits, flagging any sequence that is
 // not the image of a payload.
 //
-// A block is a sequential recursion, so one lane decodes one block and the
-// launch is parallel over blocks.  The composition (at most kMaxPoints
+// A block is a sequential recursion over its points.  Up to 16 points a
+// group of lanes (one per point, a power of two) decodes a block, splitting
+// each position's divisions and combining them with DPP moves; larger
+// alphabets take one lane per block.  The composition (at most kMaxPoints
 // points) comes in the kernel arguments; each lane keeps its counts and the
 // current split in registers (up to 16 points; larger alphabets compile to
 // a variant whose arrays the compiler keeps in scratch).  floor(a /
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(kLanes) void ccdm_match_kernel(const signed char* _
         rng = sh;
         cd.take(s);
         --remaining;
-        while (rng < kHalf) {
+        for (int g = 0; g < 32 && rng < kHalf; ++g) {  // rng >= 1: at most 31
             rng <<= 1;
             d = (d << 1) | (pos < k ? u[pos] : 0);
             ++pos;
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(kLanes) void ccdm_dematch_kernel(
         rng = cd.share_of(s);
         cd.take(s);
         --remaining;
-        while (rng < kHalf) {
+        for (int g = 0; g < 32 && rng < kHalf; ++g) {  // rng >= 1: at most 31
             rng <<= 1;
             ++shift;
         }
@@ -273,6 +275,265 @@ __global__ __launch_bounds__(kLanes) void ccdm_dematch_kernel(
     valid[b] = ok ? 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Group kernels (up to 16 points): G lanes per block, lane j holds point j's
+// count and computes its share, so a position's divisions run side by side;
+// the group's sum, argmax, prefix and broadcasts are DPP moves within the
+// lane's row of 16 (G divides 16, so a group never straddles a row).  Every
+// lane of a group follows the same positions and keeps the same copy of the
+// block's range state; lane 0 writes what the block writes.
+// ---------------------------------------------------------------------------
+
+// floor(rng * c / d) for rng <= 2^32, c <= d < 2^31: a float64 estimate,
+// then one exact 64-bit remainder check each way.
+__device__ __forceinline__ int64_t share_of_point(uint64_t rng, uint32_t c, uint32_t d, double inv) {
+    const uint64_t a = rng * c;
+    uint64_t q = static_cast<uint64_t>(static_cast<double>(rng) * (static_cast<double>(c) * inv));
+    const uint64_t qd = q * d;
+    if (qd > a) {
+        --q;
+    } else if (a - qd >= d) {
+        ++q;
+    }
+    return static_cast<int64_t>(q);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ int64_t dpp64(int64_t v) {
+    const int lo = dpp<CTRL>(static_cast<int>(static_cast<uint64_t>(v)));
+    const int hi = dpp<CTRL>(static_cast<int>(static_cast<uint64_t>(v) >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(static_cast<unsigned>(hi)) << 32) |
+                                static_cast<unsigned>(lo));
+}
+
+// The partner of each butterfly step: xor 1, xor 2 (quad permutes), the
+// mirror within 8 and within 16 lanes; after the steps up to G every lane of
+// the group holds the group's result.
+constexpr int kXor1 = 0xB1, kXor2 = 0x4E, kHalfMirror = 0x141, kMirror = 0x140;
+
+template <int G>
+__device__ __forceinline__ int64_t group_sum(int64_t v) {
+    if constexpr (G >= 2) v += dpp64<kXor1>(v);
+    if constexpr (G >= 4) v += dpp64<kXor2>(v);
+    if constexpr (G >= 8) v += dpp64<kHalfMirror>(v);
+    if constexpr (G >= 16) v += dpp64<kMirror>(v);
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ int group_max(int v) {
+    if constexpr (G >= 2) v = max(v, dpp<kXor1>(v));
+    if constexpr (G >= 4) v = max(v, dpp<kXor2>(v));
+    if constexpr (G >= 8) v = max(v, dpp<kHalfMirror>(v));
+    if constexpr (G >= 16) v = max(v, dpp<kMirror>(v));
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ int group_min(int v) {
+    if constexpr (G >= 2) v = min(v, dpp<kXor1>(v));
+    if constexpr (G >= 4) v = min(v, dpp<kXor2>(v));
+    if constexpr (G >= 8) v = min(v, dpp<kHalfMirror>(v));
+    if constexpr (G >= 16) v = min(v, dpp<kMirror>(v));
+    return v;
+}
+
+// Exclusive prefix sum over the group's lanes 0..j-1 (row shifts; a lane
+// keeps a shifted value only from its own group, j >= the shift).  Each
+// move runs with every lane of the group active and the choice comes
+// after it: a move under a narrower mask would read its source lane as
+// disabled, which returns zero.
+template <int G>
+__device__ __forceinline__ int64_t group_prefix(int64_t v, int j) {
+    int64_t x = v;
+    if constexpr (G >= 2) {
+        const int64_t y = dpp64<0x111>(x);
+        x += y & -static_cast<int64_t>(j >= 1);
+    }
+    if constexpr (G >= 4) {
+        const int64_t y = dpp64<0x112>(x);
+        x += y & -static_cast<int64_t>(j >= 2);
+    }
+    if constexpr (G >= 8) {
+        const int64_t y = dpp64<0x114>(x);
+        x += y & -static_cast<int64_t>(j >= 4);
+    }
+    if constexpr (G >= 16) {
+        const int64_t y = dpp64<0x118>(x);
+        x += y & -static_cast<int64_t>(j >= 8);
+    }
+    return x - v;
+}
+
+// One position of the coder for the group: lane j's share (the remainder
+// added at the largest count, lowest index among equals) and its start.
+template <int G>
+__device__ __forceinline__ void group_split(int cnt, int j, int64_t rng, int64_t remaining,
+                                            int64_t* share, int64_t* start) {
+    const double inv = 1.0 / static_cast<double>(remaining);
+    int64_t sh = share_of_point(static_cast<uint64_t>(rng), static_cast<uint32_t>(cnt),
+                                static_cast<uint32_t>(remaining), inv);
+    const int64_t rest = rng - group_sum<G>(sh);
+    const int cmax = group_max<G>(cnt);
+    const int big = group_min<G>(cnt == cmax ? j : G);
+    sh += (j == big) ? rest : 0;
+    *share = sh;
+    *start = group_prefix<G>(sh, j);
+}
+
+template <int G>
+__global__ __launch_bounds__(kLanes) void ccdm_match_group(const signed char* __restrict__ bits,
+                                                           long long R, long long k, Composition comp,
+                                                           int M, long long n, int* __restrict__ out) {
+    const int j = static_cast<int>(threadIdx.x) & (G - 1);
+    const long long b = (static_cast<long long>(blockIdx.x) * kLanes + threadIdx.x) / G;
+    if (b >= R) return;  // whole groups leave together
+    int cnt = j < M ? static_cast<int>(comp.count[j]) : 0;
+    const signed char* u = bits + b * k;
+    int* o = out + b * n;
+    long long pos = 0;
+    int64_t d = 0;
+    for (int i = 0; i < 32; ++i, ++pos) d = (d << 1) | (pos < k ? u[pos] : 0);
+    int64_t rng = kTop;
+    int64_t remaining = n;
+    for (long long t = 0; t < n; ++t) {
+        int64_t sh;
+        int64_t start;
+        group_split<G>(cnt, j, rng, remaining, &sh, &start);
+        const bool hit = sh > 0 && d >= start && d < start + sh;
+        const int s = group_min<G>(hit ? j : G);
+        const int64_t cum = group_sum<G>(j == s ? start : 0);
+        rng = group_sum<G>(j == s ? sh : 0);
+        if (j == 0) o[t] = s;
+        d -= cum;
+        cnt -= (j == s) ? 1 : 0;
+        --remaining;
+        for (int g = 0; g < 32 && rng < kHalf; ++g) {  // rng >= 1: at most 31
+            rng <<= 1;
+            d = (d << 1) | (pos < k ? u[pos] : 0);
+            ++pos;
+        }
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(kLanes) void ccdm_dematch_group(
+    const int* __restrict__ symbols, long long R, long long n, Composition comp, int M,
+    long long k, long long buffer_bits, signed char* __restrict__ bits,
+    unsigned char* __restrict__ valid, unsigned* __restrict__ ws) {
+    const int j = static_cast<int>(threadIdx.x) & (G - 1);
+    const long long b = (static_cast<long long>(blockIdx.x) * kLanes + threadIdx.x) / G;
+    if (b >= R) return;
+    int cnt = j < M ? static_cast<int>(comp.count[j]) : 0;
+    const long long limbs = buffer_bits / 32 + 2;
+    unsigned* low = ws + b;  // limb l at low[l * R]
+    for (long long l = j; l < limbs; l += G) low[l * R] = 0u;
+    const int* sym = symbols + b * n;
+    int64_t rng = kTop;
+    long long shift = 32;
+    int64_t remaining = n;
+    bool ok = true;
+    uint64_t hi = 0;  // limbs base - 2, base - 1
+    uint64_t lo = 0;  // limbs base, base + 1
+    long long base = -1;
+    for (long long t = 0; t < n; ++t) {
+        const int s = sym[t];
+        const bool in_range = s >= 0 && s < M;
+        const int cs = group_max<G>((in_range && j == s) ? cnt : 0);
+        if (!in_range || cs == 0) {
+            ok = false;
+            break;
+        }
+        int64_t sh;
+        int64_t start;
+        group_split<G>(cnt, j, rng, remaining, &sh, &start);
+        const int64_t cum = group_sum<G>(j == s ? start : 0);
+        const long long i = (shift - 1) >> 5;
+        while (base < i - 1) {
+            if (j == 0) emit(low, R, limbs, base - 2, hi >> 32);
+            hi = (hi << 32) | (lo >> 32);
+            lo <<= 32;
+            ++base;
+        }
+        const uint64_t v = static_cast<uint64_t>(cum) << (31 - ((shift - 1) & 31));
+        lo += v;
+        if (lo < v) {
+            hi += 1;
+            if (hi == 0 && j == 0) {
+                for (long long l = base - 3; l >= 0; --l) {
+                    const uint64_t x = static_cast<uint64_t>(low[l * R]) + 1u;
+                    low[l * R] = static_cast<unsigned>(x);
+                    if ((x >> 32) == 0) break;
+                }
+            }
+        }
+        rng = group_sum<G>(j == s ? sh : 0);
+        cnt -= (j == s) ? 1 : 0;
+        --remaining;
+        for (int g = 0; g < 32 && rng < kHalf; ++g) {  // rng >= 1: at most 31
+            rng <<= 1;
+            ++shift;
+        }
+        if (shift > buffer_bits) {
+            ok = false;
+            break;
+        }
+    }
+    if (j == 0) {
+        emit(low, R, limbs, base - 2, hi >> 32);
+        emit(low, R, limbs, base - 1, hi);
+        emit(low, R, limbs, base, lo >> 32);
+        emit(low, R, limbs, base + 1, lo);
+    }
+    // The limbs are written by lane 0 and read by the whole group below.
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // v = ceil(low 2^k).  beyond: any bit after the k-th; adding one flips
+    // the trailing ones of the first k bits and the zero before them (at
+    // position `last0`, 1-based; 0 when all k are ones: v = 2^k).
+    const long long first = k >> 5;
+    const bool beyond = (low[first * R] & (0xFFFFFFFFu >> (k & 31))) != 0u ||
+                        [&] {
+                            for (long long l = first + 1; l < limbs; ++l)
+                                if (low[l * R] != 0u) return true;
+                            return false;
+                        }();
+    long long last0 = 0;
+    if (beyond) {
+        for (long long l = (k - 1) >> 5; l >= 0; --l) {
+            unsigned w = low[l * R];
+            const long long top = l * 32 + 32;  // bit positions l*32+1 .. top
+            if (top > k) w |= 0xFFFFFFFFu >> (k - l * 32);  // ignore bits past k
+            if (w != 0xFFFFFFFFu) {
+                last0 = l * 32 + 32 - __builtin_ctz(~w);
+                break;
+            }
+        }
+    }
+    signed char* out = bits + b * k;
+    for (long long p = j + 1; p <= k; p += G) {
+        const int bit = limb_bit(low, R, p);
+        out[p - 1] = static_cast<signed char>(bit ^ ((beyond && p >= last0) ? 1 : 0));
+    }
+    if (ok && beyond && last0 == 0) ok = false;  // v = 2^k: no payload
+    if (ok && beyond) {
+        const long long lo_bit = (k + 1 > shift - 31) ? k + 1 : shift - 31;
+        for (long long p = k + 1; p < lo_bit && ok; ++p)
+            if (!limb_bit(low, R, p)) ok = false;
+        if (ok) {
+            int64_t f = 0;
+            for (long long p = lo_bit; p <= shift; ++p) f = (f << 1) | limb_bit(low, R, p);
+            ok = (int64_t{1} << (shift - lo_bit + 1)) - f < rng;
+        }
+    }
+    if (j == 0) valid[b] = ok ? 1 : 0;
+}
+
 int32_t check_composition(const int64_t* composition, int32_t num_points, int64_t n,
                           Composition* comp) {
     CK_REQUIRE(composition != nullptr, "composition is NULL");
@@ -295,6 +556,18 @@ int32_t check_composition(const int64_t* composition, int32_t num_points, int64_
 
 unsigned blocks_of(int64_t rows) { return static_cast<unsigned>((rows + kLanes - 1) / kLanes); }
 
+// Lanes per block of the group kernels (a power of two, at least the
+// points), or 0 for the one-lane kernel (more than 16 points).
+int group_of(int32_t num_points) {
+    int g = 2;
+    while (g < num_points) g *= 2;
+    return g <= 16 ? g : 0;
+}
+
+unsigned groups_of(int64_t rows, int G) {
+    return static_cast<unsigned>((rows * G + kLanes - 1) / kLanes);
+}
+
 }  // namespace
 
 extern "C" int32_t ck_ccdm_match(const int8_t* bits, int64_t rows, int64_t k,
@@ -305,18 +578,22 @@ extern "C" int32_t ck_ccdm_match(const int8_t* bits, int64_t rows, int64_t k,
     CK_REQUIRE(rows >= 0 && k >= 1, "rows %lld, k %lld", static_cast<long long>(rows),
                static_cast<long long>(k));
     if (rows == 0) return CK_OK;
-    CK_REQUIRE(rows <= (int64_t{1} << 31) * kLanes, "too many rows");
+    CK_REQUIRE(rows <= (int64_t{1} << 31), "%lld rows, at most 2^31", static_cast<long long>(rows));
     CK_REQUIRE(bits != nullptr && out != nullptr, "NULL bits or output");
     const dim3 g(blocks_of(rows)), blk(kLanes);
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const signed char* u = reinterpret_cast<const signed char*>(bits);
     const long long R = rows, kk = k, nn = n;
-    if (num_points <= 4)
-        hipLaunchKernelGGL(ccdm_match_kernel<4>, g, blk, 0, st, u, R, kk, comp, num_points, nn, out);
-    else if (num_points <= 8)
-        hipLaunchKernelGGL(ccdm_match_kernel<8>, g, blk, 0, st, u, R, kk, comp, num_points, nn, out);
-    else if (num_points <= 16)
-        hipLaunchKernelGGL(ccdm_match_kernel<16>, g, blk, 0, st, u, R, kk, comp, num_points, nn, out);
+    const int G = group_of(num_points);
+    const dim3 gg(groups_of(rows, G));
+    if (G == 2)
+        hipLaunchKernelGGL(ccdm_match_group<2>, gg, blk, 0, st, u, R, kk, comp, num_points, nn, out);
+    else if (G == 4)
+        hipLaunchKernelGGL(ccdm_match_group<4>, gg, blk, 0, st, u, R, kk, comp, num_points, nn, out);
+    else if (G == 8)
+        hipLaunchKernelGGL(ccdm_match_group<8>, gg, blk, 0, st, u, R, kk, comp, num_points, nn, out);
+    else if (G == 16)
+        hipLaunchKernelGGL(ccdm_match_group<16>, gg, blk, 0, st, u, R, kk, comp, num_points, nn, out);
     else
         hipLaunchKernelGGL(ccdm_match_kernel<kMaxPoints>, g, blk, 0, st, u, R, kk, comp, num_points, nn,
                            out);
@@ -339,7 +616,7 @@ extern "C" int32_t ck_ccdm_dematch(const int32_t* symbols, int64_t rows, int64_t
     CK_REQUIRE(buffer_bits > k, "buffer of %lld bits does not exceed k = %lld",
                static_cast<long long>(buffer_bits), static_cast<long long>(k));
     if (rows == 0) return CK_OK;
-    CK_REQUIRE(rows <= (int64_t{1} << 31) * kLanes, "too many rows");
+    CK_REQUIRE(rows <= (int64_t{1} << 31), "%lld rows, at most 2^31", static_cast<long long>(rows));
     CK_REQUIRE(symbols != nullptr && bits != nullptr && valid != nullptr && workspace != nullptr,
                "NULL symbols, bits, valid or workspace");
     const dim3 g(blocks_of(rows)), blk(kLanes);
@@ -347,14 +624,19 @@ extern "C" int32_t ck_ccdm_dematch(const int32_t* symbols, int64_t rows, int64_t
     signed char* out = reinterpret_cast<signed char*>(bits);
     unsigned* ws = static_cast<unsigned*>(workspace);
     const long long R = rows, nn = n, kk = k, bb = buffer_bits;
-    if (num_points <= 4)
-        hipLaunchKernelGGL(ccdm_dematch_kernel<4>, g, blk, 0, st, symbols, R, nn, comp, num_points,
+    const int G = group_of(num_points);
+    const dim3 gg(groups_of(rows, G));
+    if (G == 2)
+        hipLaunchKernelGGL(ccdm_dematch_group<2>, gg, blk, 0, st, symbols, R, nn, comp, num_points,
                            kk, bb, out, valid, ws);
-    else if (num_points <= 8)
-        hipLaunchKernelGGL(ccdm_dematch_kernel<8>, g, blk, 0, st, symbols, R, nn, comp, num_points,
+    else if (G == 4)
+        hipLaunchKernelGGL(ccdm_dematch_group<4>, gg, blk, 0, st, symbols, R, nn, comp, num_points,
                            kk, bb, out, valid, ws);
-    else if (num_points <= 16)
-        hipLaunchKernelGGL(ccdm_dematch_kernel<16>, g, blk, 0, st, symbols, R, nn, comp, num_points,
+    else if (G == 8)
+        hipLaunchKernelGGL(ccdm_dematch_group<8>, gg, blk, 0, st, symbols, R, nn, comp, num_points,
+                           kk, bb, out, valid, ws);
+    else if (G == 16)
+        hipLaunchKernelGGL(ccdm_dematch_group<16>, gg, blk, 0, st, symbols, R, nn, comp, num_points,
                            kk, bb, out, valid, ws);
     else
         hipLaunchKernelGGL(ccdm_dematch_kernel<kMaxPoints>, g, blk, 0, st, symbols, R, nn, comp,
