@@ -646,7 +646,7 @@ def main() -> int:
         return 2
     import torch
 
-    from commkit_amd import Constellation, MinSum
+    from commkit_amd import Constellation, MinSum, metrics
     from commkit_amd.coding.ldpc import _decode
     from commkit_amd.mapping.demap import llr_any
     from commkit_amd.operating import waterfall_db
@@ -704,9 +704,9 @@ def main() -> int:
 
     bits, app, valid, iters = out
     iters_h = iters.cpu().numpy()
-    errors = (bits != fr.info).sum(dim=1).cpu().numpy()
-    ber = float(errors.sum() / (F * code.k))
-    fer = float((errors > 0).mean())
+    # Scored by the library's own kernels, after the clock stopped.
+    ber = metrics.ber(bits.reshape(-1), fr.info.reshape(-1))
+    fer = metrics.fer(bits.reshape(-1), fr.info.reshape(-1), info_bits=code.k)
     info_bits = F * code.k
     value = args.steps * info_bits / elapsed / 1e9
     dec_bytes = decoder_bytes(code, iters_h)

@@ -11,7 +11,7 @@ array), and the results stay on the input's device.
 
 from ._array import PrecisionWarning
 from .coding import DecodeResult, LDPCCode, MinSum, SumProduct, decode, encode
-from .core import Signal
+from .core import Reference, Signal
 from .mapping import Constellation, compute_llr
 
 __all__ = [
@@ -20,6 +20,7 @@ __all__ = [
     "LDPCCode",
     "MinSum",
     "PrecisionWarning",
+    "Reference",
     "Signal",
     "SumProduct",
     "compute_llr",

@@ -1,5 +1,5 @@
 """Signal records read by the receive tail."""
 
-from .signal import Signal
+from .signal import Reference, Signal
 
-__all__ = ["Signal"]
+__all__ = ["Reference", "Signal"]

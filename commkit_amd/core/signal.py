@@ -4,7 +4,9 @@
 46,216-240,309-327``): a Signal must be at one sample per symbol and carry
 no frame, and supplies the constellation when the call does not.  Only the
 fields that contract reads are kept here: the samples, the constellation,
-samples per symbol and the frame.
+samples per symbol and the frame, plus the reference of what was sent,
+which scoring (:mod:`commkit_amd.metrics`) and storage (:mod:`commkit_amd.io`)
+read.
 """
 
 from __future__ import annotations
@@ -12,7 +14,24 @@ from __future__ import annotations
 from dataclasses import dataclass
 from typing import Any
 
-__all__ = ["Signal", "signal_input"]
+__all__ = ["Reference", "Signal", "signal_input"]
+
+
+@dataclass(frozen=True, eq=False)
+class Reference:
+    """What was sent (``commkit/core/signal.py``'s Reference, as far as scoring
+    and storage read it).
+
+    symbols : the transmitted symbols.
+    bits : the bits on the air (the labels of ``symbols``).
+    info_bits : the source bits before ``code`` (``bits`` when uncoded).
+    code : the code between them (LDPCCode, CCDM, PAS) or ``None``.
+    """
+
+    symbols: Any = None
+    bits: Any = None
+    info_bits: Any = None
+    code: Any = None
 
 
 @dataclass(frozen=True, eq=False)
@@ -23,12 +42,14 @@ class Signal:
     constellation : the transmitted :class:`~commkit_amd.mapping.Constellation`.
     sps : samples per symbol (the tail needs 1).
     frame : framing of the record, ``None`` when the samples are payload only.
+    reference : the :class:`Reference` of what was sent, or ``None``.
     """
 
     samples: Any
     constellation: Any = None
     sps: int = 1
     frame: Any = None
+    reference: Any = None
 
 
 def signal_input(symbols: Any, constellation: Any, function_name: str) -> tuple[Any, Any]:

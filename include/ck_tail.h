@@ -353,6 +353,24 @@ int32_t ck_ccdm_dematch(const int32_t* symbols, int64_t rows, int64_t n,
                         int64_t buffer_bits, int8_t* bits, uint8_t* valid,
                         void* workspace, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Post-FEC scoring.  Device arrays; the outputs are cleared by the call.
+ * ------------------------------------------------------------------- */
+
+/* Per row of `bits` and `reference` (rows, n) int8 0/1: bit_errors[r] = the
+ * differing bits, block_errors[r] = the blocks of `block` bits (block
+ * divides n) with any: ber and fer (commkit/metrics/error_rates.py:30-181). */
+int32_t ck_bit_errors(const int8_t* bits, const int8_t* reference, int64_t rows,
+                      int64_t n, int64_t block, int64_t* bit_errors,
+                      int64_t* block_errors, void* stream);
+
+/* Per row of `llrs` (rows, n) float32 and the sent `bits` (rows, n) int8:
+ * loss[r] = mean over the row of log2(1 + e^{-(1 - 2c) L}) (float64), the
+ * term behind gmi = H(X) - k loss and ngmi = 1 - loss
+ * (commkit/metrics/information.py:34-89). */
+int32_t ck_gmi_loss(const float* llrs, const int8_t* bits, int64_t rows, int64_t n,
+                    double* loss, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
