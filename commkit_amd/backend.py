@@ -113,9 +113,9 @@ def device_tables(
         hit = cache.get(key)
         if hit is not None:
             return hit
-    tables = tuple(
-        torch.from_numpy(np.ascontiguousarray(a)).to(device) for a in build()
-    )
+    # np.array copies: read-only host tables (frozen value objects) would
+    # otherwise be wrapped as non-writable tensors before the upload.
+    tables = tuple(torch.from_numpy(np.array(a, order="C")).to(device) for a in build())
     if owner is not None:
         cache[key] = tables
     return tables

@@ -25,7 +25,8 @@ import math
 from typing import Any
 
 __all__ = ["WATERFALL_DB", "SHAPED_WATERFALL_DB", "waterfall_db", "calibrate",
-           "calibrate_shaped", "frame_error_rate", "shaped_frame_error_rate"]
+           "calibrate_shaped", "frame_error_rate", "shaped_frame_error_rate",
+           "threshold_esn0_db"]
 
 #: Measured waterfalls (dB) by (code name, constellation order), from
 #: ``python -m commkit_amd.operating`` on an MI355X.
@@ -161,9 +162,56 @@ def calibrate(code_name: str, order: int, *, start_db: float | None = None,
     return round(es, 2)
 
 
+def threshold_esn0_db(code: Any, constellation: Any, *, tol: float = 1e-3) -> float:
+    """The Es/N0 (dB) at which the BICM channel of ``constellation`` carries
+    the code's belief-propagation threshold: where the NGMI,
+    ``1 - (H(X) - GMI) / m``, equals :func:`~commkit_amd.coding.exit.
+    ldpc_threshold` (bisection to ``tol`` dB, the GMI by
+    :func:`commkit_amd.metrics.awgn_gmi` on the GPU).
+
+    The threshold is a long-code, sum-product, Gaussian-approximation
+    prediction (``commkit/coding/exit.py:720-780``): the measured waterfalls
+    (finite codewords, min-sum, at most 50 iterations, FER 1e-2) lie above
+    it, and the gap is what those choices cost.
+    """
+    from .coding.exit import ldpc_threshold
+    from .metrics import awgn_gmi
+
+    thr = ldpc_threshold(code)
+    if math.isnan(thr):
+        return math.nan
+    m = constellation.bits_per_symbol
+    h = constellation.entropy
+
+    def ngmi(e: float) -> float:
+        return 1.0 - (h - float(awgn_gmi(constellation, esn0_db=e))) / m
+
+    lo, hi = -10.0, 40.0
+    while hi - lo > tol:
+        mid = 0.5 * (lo + hi)
+        if ngmi(mid) >= thr:
+            hi = mid
+        else:
+            lo = mid
+    return hi
+
+
 if __name__ == "__main__":
     import sys
 
+    if sys.argv[1:] == ["--thresholds"]:
+        from .coding.exit import ldpc_threshold
+        from .mapping import Constellation
+        from .workloads import seeded_code
+
+        print("code       QAM  rate   threshold  predicted dB  measured dB  gap dB")
+        for (name, order), wf in sorted(WATERFALL_DB.items()):
+            code = seeded_code(name)
+            thr = ldpc_threshold(code)
+            pred = threshold_esn0_db(code, Constellation.qam(order))
+            print(f"{name:10s} {order:4d}  {code.k / code.n:.3f}  {thr:.4f}     {pred:7.2f}"
+                  f"       {wf:7.2f}     {wf - pred:5.2f}", flush=True)  # fmt: skip
+        sys.exit(0)
     pairs = [("r4/5-z360", 64), ("r1/2-z27", 16)]
     if len(sys.argv) > 1:
         pairs = [tuple(a.split(":")) for a in sys.argv[1:]]  # type: ignore[misc]
