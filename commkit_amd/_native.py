@@ -57,6 +57,8 @@ SYMBOLS: dict[str, tuple[Any, list[Any]]] = {
     "ck_ccdm_dematch": (_i32, [_p, _i64, _i64, _p, _i32, _i64, _i64, _p, _p, _p, _p]),
     "ck_bit_errors": (_i32, [_p, _p, _i64, _i64, _i64, _p, _p, _p]),
     "ck_gmi_loss": (_i32, [_p, _p, _i64, _i64, _p, _p]),
+    "ck_ldpc_encode": (_i32, [_p, _p, _i64, _p, _p]),
+    "ck_awgn": (_i32, [_p, _i64, _i64, _i32, _p, ctypes.c_uint64, ctypes.c_uint64, _p, _p]),
 }
 
 _LIB: ctypes.CDLL | None = None

@@ -427,42 +427,76 @@ def _build_encoder(code: LDPCCode) -> _Encoder:
 # -----------------------------------------------------------------------------
 
 
-def _encode(code: LDPCCode, bits: Any) -> Any:
-    """Encode ``(R, k)`` int8 device rows into ``(R, n)`` int8 codewords."""
+class CkLdpcEncoder(ctypes.Structure):
+    """``ck_ldpc_encoder`` of ``include/ck_tail.h``."""
+
+    _fields_ = [
+        ("k", ctypes.c_int32),
+        ("m", ctypes.c_int32),
+        ("n_out", ctypes.c_int32),
+        ("words", ctypes.c_int32),
+        ("info_row_ptr", ctypes.c_void_p),
+        ("info_cols", ctypes.c_void_p),
+        ("inverse_t", ctypes.c_void_p),
+    ]
+
+
+def _encoder_tables(code: LDPCCode) -> tuple[np.ndarray, ...]:
+    """Host tables of ``ck_ldpc_encode``: the CSR of H over the information
+    columns, and Hp^-1 packed 64 columns a word, transposed to (words, m)
+    (absent for a dual-diagonal Hp)."""
+    enc = code._encoder
+    m, k = code.num_checks, code.k
+    info_rows, info_cols = enc.tables[0], enc.tables[1]  # sorted by (row, col)
+    row_ptr = np.concatenate([[0], np.cumsum(np.bincount(info_rows, minlength=m))])
+    words = (m + 63) // 64
+    tables = [row_ptr.astype(np.int32), np.asarray(info_cols, np.int32)]
+    if enc.kind == "inverse":
+        inv = np.zeros((m, words * 64), dtype=np.uint8)
+        inv[:, :m] = enc.tables[2]
+        packed = np.packbits(inv.reshape(m, words, 64), axis=-1, bitorder="little")
+        tables.append(np.ascontiguousarray(packed.view("<u8").reshape(m, words).T))
+    return tuple(tables)
+
+
+def _encode_hip(code: LDPCCode, bits: Any) -> Any:
+    """``ck_ldpc_encode`` of ``(R, k)`` int8 device rows into ``(R, n)``."""
     import torch
 
     enc = code._encoder
-    m, k, N = code.num_checks, code.k, code.num_variables
     dev = bits.device
-    u = bits.to(torch.float32)
-    if enc.kind == "dense":
-        (g,) = device_tables(code, "enc_dense", dev, lambda: (enc.tables[0].astype(np.float32),))
-        parity = (u @ g).to(torch.int64) & 1
-    else:
-        info_rows, info_cols = enc.tables[0], enc.tables[1]
+    tabs = device_tables(code, "enc_hip", dev, lambda: _encoder_tables(code))
+    m = code.num_checks
+    desc = CkLdpcEncoder(code.k, m, code.n, (m + 63) // 64, tabs[0].data_ptr(),
+                         tabs[1].data_ptr(), tabs[2].data_ptr() if enc.kind == "inverse" else None)  # fmt: skip
+    u = bits.to(torch.int8).contiguous()
+    out = torch.empty((u.shape[0], code.n), dtype=torch.int8, device=dev)
+    with torch.cuda.device(dev):
+        _native.check(_native.lib().ck_ldpc_encode(ctypes.byref(desc), u.data_ptr(), u.shape[0],
+                                                   out.data_ptr(), _native.stream_of(dev)))  # fmt: skip
+    return out
 
-        def build_hi() -> tuple[np.ndarray]:
-            hi = np.zeros((m, k), dtype=np.float32)
-            hi[info_rows, info_cols] = 1.0
-            return (hi,)
 
-        (hi,) = device_tables(code, "enc_hi", dev, build_hi)
-        s = (u @ hi.T).to(torch.int64) & 1  # (R, m): syndrome of the info part
-        if enc.kind == "accumulate":
-            parity = torch.cumsum(s, dim=1) & 1
-        else:
-            (inv_t,) = device_tables(
-                code, "enc_inv", dev, lambda: (enc.tables[2].T.astype(np.float32),)
-            )
-            parity = (s.to(torch.float32) @ inv_t).to(torch.int64) & 1
-    words = torch.empty((bits.shape[0], N), dtype=torch.int8, device=dev)
-    if enc.kind == "dense":
-        info_t, par_t = device_tables(code, "enc_pos", dev, lambda: (enc.info, enc.parity))
-        words[:, info_t] = bits.to(torch.int8)
-        words[:, par_t] = parity.to(torch.int8)
-    else:
-        words[:, :k] = bits.to(torch.int8)
-        words[:, k:] = parity.to(torch.int8)
+def _encode(code: LDPCCode, bits: Any) -> Any:
+    """Encode ``(R, k)`` int8 device rows into ``(R, n)`` int8 codewords:
+    ``ck_ldpc_encode`` for a systematic code (invertible or dual-diagonal
+    parity part), a GF(2) product with the generator otherwise."""
+    import torch
+
+    if not bits.is_cuda:
+        raise ValueError("the LDPC encoder takes device rows; encode() uploads host input.")
+    enc = code._encoder
+    if enc.kind in ("inverse", "accumulate"):
+        return _encode_hip(code, bits)
+    # Dense: the systematic generator of the row-reduced H (float32 sums of
+    # at most k ones are exact), information and parity at their positions.
+    dev = bits.device
+    (g,) = device_tables(code, "enc_dense", dev, lambda: (enc.tables[0].astype(np.float32),))
+    parity = (bits.to(torch.float32) @ g).to(torch.int64) & 1
+    words = torch.empty((bits.shape[0], code.num_variables), dtype=torch.int8, device=dev)
+    info_t, par_t = device_tables(code, "enc_pos", dev, lambda: (enc.info, enc.parity))
+    words[:, info_t] = bits.to(torch.int8)
+    words[:, par_t] = parity.to(torch.int8)
     return words[:, : code.n]
 
 

@@ -1,5 +1,6 @@
-"""Signal records read by the receive tail."""
+"""Signal records read by the receive tail, and random payloads."""
 
+from .generation import generate
 from .signal import Reference, Signal
 
-__all__ = ["Reference", "Signal"]
+__all__ = ["Reference", "Signal", "generate"]

@@ -43,6 +43,7 @@ class Signal:
     sps : samples per symbol (the tail needs 1).
     frame : framing of the record, ``None`` when the samples are payload only.
     reference : the :class:`Reference` of what was sent, or ``None``.
+    symbol_rate : symbols per second, or ``None`` when not recorded.
     """
 
     samples: Any
@@ -50,6 +51,7 @@ class Signal:
     sps: int = 1
     frame: Any = None
     reference: Any = None
+    symbol_rate: float | None = None
 
 
 def signal_input(symbols: Any, constellation: Any, function_name: str) -> tuple[Any, Any]:

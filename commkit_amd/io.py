@@ -79,9 +79,10 @@ def save_npz(signal: Signal, path: str | Path, *, compressed: bool = True) -> No
     if path.suffix != ".npz":
         path = path.with_name(path.name + ".npz")
     arrays: dict[str, Any] = {"samples": _host(signal.samples)}
+    rate = 1.0 if signal.symbol_rate is None else float(signal.symbol_rate)
     meta: dict[str, Any] = {
-        "sampling_rate": float(signal.sps),
-        "symbol_rate": 1.0,
+        "sampling_rate": rate * signal.sps,
+        "symbol_rate": rate,
         "center_frequency": 0.0,
         "domain": "time",
         "pulse": None,
@@ -152,6 +153,7 @@ def _read_signal(data: Any, path: Path) -> Signal:
         constellation=_get_constellation("constellation", data, meta),
         sps=int(round(sps)),
         reference=reference,
+        symbol_rate=None if sr is None else float(sr),
     )
 
 
@@ -169,8 +171,7 @@ def _to_gpu(sig: Signal) -> Signal:
     if ref is not None:
         ref = Reference(symbols=up(ref.symbols), bits=up(ref.bits),
                         info_bits=up(ref.info_bits), code=ref.code)  # fmt: skip
-    return Signal(samples=up(sig.samples), constellation=sig.constellation, sps=sig.sps,
-                  frame=None, reference=ref)  # fmt: skip
+    return dataclasses.replace(sig, samples=up(sig.samples), reference=ref)
 
 
 def _put_constellation(c: Any, key: str, arrays: dict, meta: dict) -> None:

@@ -371,6 +371,45 @@ int32_t ck_bit_errors(const int8_t* bits, const int8_t* reference, int64_t rows,
 int32_t ck_gmi_loss(const float* llrs, const int8_t* bits, int64_t rows, int64_t n,
                     double* loss, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Device front end: LDPC encoding and AWGN.
+ * ------------------------------------------------------------------- */
+
+/* A systematic LDPC encoder: codeword = [u, p] with the syndrome of the
+ * information part s = H_info u and the parity p = Hp^-1 s over GF(2)
+ * (commkit/coding/ldpc.py:445-536 builds the same systematic form). Device
+ * tables. */
+typedef struct ck_ldpc_encoder {
+    int32_t k;                    /* information bits                        */
+    int32_t m;                    /* checks = parity bits of the mother code */
+    int32_t n_out;                /* bits written per codeword: k + m - punctured */
+    int32_t words;                /* 64-bit words of a syndrome, ceil(m / 64) */
+    const int32_t* info_row_ptr;  /* (m + 1) CSR of H over the info columns  */
+    const int32_t* info_cols;     /* their columns, < k                      */
+    const uint64_t* inverse_t;    /* (words, m): bit b of [w][j] is
+                                     (Hp^-1)[j][64 w + b]; NULL for a
+                                     dual-diagonal Hp (p = prefix XOR of s)  */
+} ck_ldpc_encoder;
+
+/* Encode `rows` codewords: info (rows, k) int8 0/1 to words (rows, n_out)
+ * int8.  Replaces the encoder of commkit/coding/ldpc.py:445-536 for codes
+ * with an invertible or dual-diagonal parity part. */
+int32_t ck_ldpc_encode(const ck_ldpc_encoder* enc, const int8_t* info, int64_t rows,
+                       int8_t* words, void* stream);
+
+/* out = x + noise_std[row] * n with n standard normal (circular complex of
+ * unit variance per component for complex samples), x (rows, len) float32
+ * or interleaved complex64 (is_complex), noise_std (rows) float32 device.  Normals
+ * come from Philox4x32-10 (key = seed, counter = (group, stream)), four
+ * uniforms per call turned into four normals by Box-Muller; element e of
+ * the flat array takes normal e of the draw (real), or normals 2e and
+ * 2e + 1 (complex), so results depend only on (seed, stream) and the
+ * element's index.  out may alias x.  The draws of commkit/_random.py:29-60
+ * and commkit/impairments/noise.py:638-672, on the device. */
+int32_t ck_awgn(const void* x, int64_t rows, int64_t len, int32_t is_complex,
+                const float* noise_std, uint64_t seed, uint64_t stream_id, void* out,
+                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

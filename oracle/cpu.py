@@ -1,7 +1,8 @@
 """ctypes wrappers of ``libck_oracle.so`` -- TEST INFRASTRUCTURE ONLY.
 
 NumPy in, NumPy out.  Each wrapper names the C function (and through it the
-reference lines it restates, listed in ``oracle/ck_oracle.c``).
+reference lines it restates, listed in ``oracle/ck_oracle.c``); the device
+random stream of the AWGN kernel is restated in NumPy at the end.
 """
 
 from __future__ import annotations
@@ -173,3 +174,46 @@ def ccdm_dematch(symbols: np.ndarray, composition: np.ndarray, k: int,
     lib().oracle_ccdm_dematch(_ptr(sym), sym.shape[0], sym.shape[1], _ptr(comp), comp.size, k,
                               buffer_bits, _ptr(bits), _ptr(valid))  # fmt: skip
     return bits, valid.astype(bool)
+
+
+# -- the device random stream (NumPy restatement) ---------------------------------
+
+_PHILOX_M = (np.uint64(0xD2511F53), np.uint64(0xCD9E8D57))
+_PHILOX_W = (np.uint64(0x9E3779B9), np.uint64(0xBB67AE85))
+_MASK32 = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(counter: np.ndarray, key: tuple[int, int]) -> np.ndarray:
+    """Philox4x32-10 (Salmon et al., SC 2011) of (N, 4) uint32 counters under a
+    two-word key: ten rounds of two 32x32->64 products, the high halves
+    mixed with the other words and the key, the key bumped by the Weyl
+    constants.  Pinned by the published known-answer vectors."""
+    c = [np.asarray(counter, dtype=np.uint64)[:, i].copy() for i in range(4)]
+    k0, k1 = np.uint64(key[0]), np.uint64(key[1])
+    for _ in range(10):
+        p0 = _PHILOX_M[0] * c[0]
+        p1 = _PHILOX_M[1] * c[2]
+        c = [((p1 >> np.uint64(32)) ^ c[1] ^ k0) & _MASK32, p1 & _MASK32,
+             ((p0 >> np.uint64(32)) ^ c[3] ^ k1) & _MASK32, p0 & _MASK32]
+        k0 = (k0 + _PHILOX_W[0]) & _MASK32
+        k1 = (k1 + _PHILOX_W[1]) & _MASK32
+    return np.stack(c, axis=1).astype(np.uint32)
+
+
+def awgn_normals(seed: int, stream: int, count: int) -> np.ndarray:
+    """The first ``count`` standard normals of ``ck_awgn``'s stream, in
+    float64: draw g has counter (g lo, g hi, stream lo, stream hi) and key
+    (seed lo, seed hi); uniforms (x >> 8 + 1/2) 2^-24; Box-Muller on the
+    pairs (0, 1) and (2, 3)."""
+    groups = -(-count // 4)
+    g = np.arange(groups, dtype=np.uint64)
+    ctr = np.stack([g & _MASK32, g >> np.uint64(32),
+                    np.full(groups, stream & 0xFFFFFFFF, np.uint64),
+                    np.full(groups, stream >> 32, np.uint64)], axis=1)  # fmt: skip
+    x = philox4x32_10(ctr, (seed & 0xFFFFFFFF, seed >> 32)).astype(np.float64)
+    u = (np.floor(x / 256.0) + 0.5) * 2.0**-24
+    r0 = np.sqrt(-2.0 * np.log(u[:, 0]))
+    r1 = np.sqrt(-2.0 * np.log(u[:, 2]))
+    z = np.stack([r0 * np.cos(2 * np.pi * u[:, 1]), r0 * np.sin(2 * np.pi * u[:, 1]),
+                  r1 * np.cos(2 * np.pi * u[:, 3]), r1 * np.sin(2 * np.pi * u[:, 3])], axis=1)  # fmt: skip
+    return z.reshape(-1)[:count]

@@ -58,12 +58,28 @@ def rows_in_schedule(code: LDPCCode) -> list[list[int]]:
 
 
 def encode_host(code: LDPCCode, u: np.ndarray) -> np.ndarray:
-    """Encode (R, k) bits with the library's encoder on CPU torch tensors."""
-    import torch
+    """Encode (R, k) bits on the host in NumPy from the encoder tables the
+    library builds (information syndrome, then the inverse, the accumulator
+    or the dense generator); codewords (R, n) int8."""
+    enc = code._encoder
+    u = np.asarray(u, dtype=np.int64)
+    m, k, n_mother = code.num_checks, code.k, code.num_variables
+    if enc.kind == "dense":
+        parity = (u @ enc.tables[0].astype(np.int64)) % 2
+        words = np.zeros((u.shape[0], n_mother), dtype=np.int8)
+        words[:, enc.info] = u
+        words[:, enc.parity] = parity
+        return words[:, : code.n]
+    from scipy import sparse
 
-    from commkit_amd.coding.ldpc import _encode
-
-    return _encode(code, torch.from_numpy(np.ascontiguousarray(u, np.int8))).numpy()
+    h_info = sparse.csr_matrix((np.ones(enc.tables[0].size, np.int64),
+                                (enc.tables[0], enc.tables[1])), shape=(m, k))  # fmt: skip
+    s = np.asarray(h_info @ u.T).T % 2
+    if enc.kind == "accumulate":
+        parity = np.cumsum(s, axis=1) % 2
+    else:
+        parity = (s @ enc.tables[2].astype(np.int64).T) % 2
+    return np.concatenate([u, parity], axis=1).astype(np.int8)[:, : code.n]
 
 
 def syndrome(code: LDPCCode, words: np.ndarray) -> np.ndarray:

@@ -217,3 +217,25 @@ def test_early_stop_semantics(flood):
     noise = np.random.default_rng(8).standard_normal((4, code.n)).astype(np.float32)
     _, iters, valid = run(noise)
     assert not valid.any() and (iters == 6).all()
+
+
+def test_philox_known_answers():
+    """The published Philox4x32-10 known-answer vectors (Random123)."""
+    ctr = np.array([[0, 0, 0, 0], [0xFFFFFFFF] * 4,
+                    [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]], dtype=np.uint32)  # fmt: skip
+    keys = [(0, 0), (0xFFFFFFFF, 0xFFFFFFFF), (0xA4093822, 0x299F31D0)]
+    want = [[0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8],
+            [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD],
+            [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]]  # fmt: skip
+    for c, k, w in zip(ctr, keys, want):
+        np.testing.assert_array_equal(cpu.philox4x32_10(c[None, :], k)[0], w)
+
+
+def test_awgn_normals_are_standard():
+    """2^20 normals of the restated stream: mean 0, variance 1, uncorrelated
+    pairs, within 5 standard errors."""
+    z = cpu.awgn_normals(seed=12345, stream=7, count=1 << 20)
+    se = 1.0 / np.sqrt(z.size)
+    assert abs(z.mean()) < 5 * se
+    assert abs(z.var() - 1.0) < 5 * np.sqrt(2) * se
+    assert abs(np.mean(z[0::2] * z[1::2])) < 5 * np.sqrt(2) * se
