@@ -202,6 +202,50 @@ class LDPCCode:
         _warn_if_4_cycles(code)
         return code
 
+    @classmethod
+    def dvbs2(cls, rate: str, *, frame: str = "normal") -> "LDPCCode":
+        """DVB-S2 LDPC code (ETSI EN 302 307-1 clause 5.3.2, Annexes B and C),
+        without the outer BCH code (``commkit/coding/ldpc.py:217-265``).
+
+        ``frame`` "normal" (n = 64800) or "short" (16200).  Information bit
+        ``360 g + j`` checks rows ``(x + j q) mod m`` for each address ``x``
+        of group ``g``, ``q = m / 360``; parity bit ``i`` checks rows ``i``
+        and ``i + 1`` (the accumulator, so the last parity bit has degree
+        1).  Five short labels carry fewer bits than ``n * rate`` (``"1/2"``
+        short has k = 7200): read ``code.k``.
+        """
+        from .standard_tables import DVBS2
+
+        n = {"normal": 64800, "short": 16200}.get(frame)
+        if n is None:
+            raise ValueError(f"frame must be 'normal' or 'short', got {frame!r}.")
+        text = DVBS2.get((frame, rate))
+        if text is None:
+            rates = [r for f, r in DVBS2 if f == frame]
+            raise ValueError(f"rate must be one of {rates} for {frame} frames, got {rate!r}.")
+        groups = [np.array(line.split(), dtype=np.int64) for line in text.strip().splitlines()]
+        k = 360 * len(groups)
+        m = n - k
+        q = m // 360
+        j = np.arange(360)
+        rows = [((x[:, None] + j[None, :] * q) % m).ravel() for x in groups]
+        cols = [np.repeat(360 * g + j[None, :], x.size, axis=0).ravel()
+                for g, x in enumerate(groups)]  # fmt: skip
+        p = np.arange(m)
+        rows += [p, p[1:]]
+        cols += [k + p, k + p[:-1]]
+        return cls(np.concatenate(rows), np.concatenate(cols), m, n)
+
+    @classmethod
+    def pon(cls) -> "LDPCCode":
+        """LDPC(17280, 14592) of IEEE 802.3ca / ITU-T G.9804.2 Annex B.1: the
+        12 x 69 quasi-cyclic mother code with circulants of 256 (N = 17664),
+        the last 384 parity bits punctured (``commkit/coding/ldpc.py:267-290``)."""
+        from .standard_tables import PON_BASE
+
+        base = np.array([line.split() for line in PON_BASE.strip().splitlines()], dtype=np.int64)
+        return cls.quasi_cyclic(base, lifting=256, punctured=384)
+
     # -- cached structure --------------------------------------------------------
 
     @cached_property

@@ -233,20 +233,168 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
 // Generic CSR kernel
 // ---------------------------------------------------------------------------
 
+// MAXD > 0: rows of at most MAXD edges, whose columns and values are loaded
+// as one batch into registers (a row then costs one memory round trip for
+// its columns and one for its values, instead of two per edge); MAXD = 0:
+// any degree, edge by edge.
+template <int MAXD>
 __device__ bool generic_syndrome_ok(const ck_ldpc_code& c, const float* a) {
-    int bad = 0;
-    for (int row = threadIdx.x; row < c.num_checks; row += blockDim.x) {
+    // Rounds of blockDim rows; the check stops at the first round with an
+    // unsatisfied row (the frame is not a codeword), which while decoding
+    // is usually the first.
+    for (int base = 0; base < c.num_checks; base += blockDim.x) {
+        const int row = base + threadIdx.x;
         int parity = 0;
-        for (int e = c.row_ptr[row]; e < c.row_ptr[row + 1]; ++e) parity ^= a[c.cols[e]] < 0.0f;
-        bad |= parity;
+        if (row < c.num_checks) {
+            const int e0 = c.row_ptr[row];
+            const int e1 = c.row_ptr[row + 1];
+            if (MAXD > 0) {
+                // unconditional loads from clamped addresses (see row_load_cols)
+                constexpr int D = MAXD > 0 ? MAXD : 1;
+                const int last = e1 > e0 ? e1 - 1 : 0;
+                int col[D];
+                float v[D];
+#pragma unroll
+                for (int i = 0; i < D; ++i) col[i] = c.cols[e0 + i < e1 ? e0 + i : last];
+#pragma unroll
+                for (int i = 0; i < D; ++i) v[i] = a[col[i]];
+#pragma unroll
+                for (int i = 0; i < D; ++i) parity ^= (e0 + i < e1 && v[i] < 0.0f) ? 1 : 0;
+            } else {
+                for (int e = e0; e < e1; ++e) parity ^= a[c.cols[e]] < 0.0f;
+            }
+        }
+        if (__syncthreads_or(parity)) return false;
     }
-    return !__syncthreads_or(bad);
+    return true;
 }
 
-template <bool SP, bool UPDATE_APP>
+// A row of at most MAXD edges with its columns and Q values in registers:
+// the arithmetic and its order are those of generic_row below (Q = APP - R
+// read once; within a layer no other row touches the row's columns, so the
+// second pass's APP equals the first's).  Loading and updating are split so
+// that a lane can put the loads of two rows in flight before either update.
+template <int MAXD>
+struct RowRegs {
+    int e0;
+    int col[MAXD];
+    float q[MAXD];
+};
+
+// The loads are unconditional, from clamped in-bounds addresses (the row's
+// last edge, or edge / variable 0 for no row), and the padding slots are
+// marked afterwards: loads under per-slot conditions would each be waited
+// for before the next is issued.
+template <int MAXD>
+__device__ __forceinline__ void row_load_cols(const ck_ldpc_code& c, int row, RowRegs<MAXD>& r) {
+    const int e0 = row >= 0 ? c.row_ptr[row] : 0;
+    const int e1 = row >= 0 ? c.row_ptr[row + 1] : 0;
+    r.e0 = e0;
+    const int last = e1 > e0 ? e1 - 1 : 0;
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i) {
+        const int v = c.cols[e0 + i < e1 ? e0 + i : last];
+        r.col[i] = e0 + i < e1 ? v : -1;
+    }
+}
+
+template <int MAXD>
+__device__ __forceinline__ void row_load_q(const float* a, const float* __restrict__ Rm,
+                                           bool first, RowRegs<MAXD>& r) {
+    float av[MAXD];
+    float rv[MAXD];
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i) {
+        const bool in = r.col[i] >= 0;
+        av[i] = a[in ? r.col[i] : 0];
+        rv[i] = first ? 0.0f : Rm[in ? r.e0 + i : r.e0];
+    }
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i) r.q[i] = av[i] - rv[i];
+}
+
+template <bool SP, bool UPDATE_APP, int MAXD>
+__device__ __forceinline__ void row_update(float* a, float* __restrict__ Rm, const Params& prm,
+                                           const RowRegs<MAXD>& r) {
+    unsigned neg = 0u;
+    if (SP) {
+        float total = 0.0f;
+#pragma unroll
+        for (int i = 0; i < MAXD; ++i)
+            if (r.col[i] >= 0) {
+                neg ^= (r.q[i] < 0.0f) ? 1u : 0u;
+                total = total + phi(fabsf(r.q[i]));
+            }
+#pragma unroll
+        for (int i = 0; i < MAXD; ++i)
+            if (r.col[i] >= 0) {
+                const float mag = phi(total - phi(fabsf(r.q[i])));
+                const float rn = (neg ^ ((r.q[i] < 0.0f) ? 1u : 0u)) ? -mag : mag;
+                if (UPDATE_APP) a[r.col[i]] = r.q[i] + rn;
+                Rm[r.e0 + i] = rn;
+            }
+        return;
+    }
+    float min1 = ck::kInf;
+    float min2 = ck::kInf;
+    int idx = -1;
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i)
+        if (r.col[i] >= 0) {
+            neg ^= (r.q[i] < 0.0f) ? 1u : 0u;
+            const float mag = fabsf(r.q[i]);
+            if (mag < min1) {
+                min2 = min1;
+                min1 = mag;
+                idx = i;
+            } else if (mag < min2) {
+                min2 = mag;
+            }
+        }
+    const float m1 = fmaxf(prm.scale * min1 - prm.offset, 0.0f);
+    const float m2 = fmaxf(prm.scale * min2 - prm.offset, 0.0f);
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i)
+        if (r.col[i] >= 0) {
+            const float mag = (i == idx) ? m2 : m1;
+            const float rn = (neg ^ ((r.q[i] < 0.0f) ? 1u : 0u)) ? -mag : mag;
+            if (UPDATE_APP) a[r.col[i]] = r.q[i] + rn;
+            Rm[r.e0 + i] = rn;
+        }
+}
+
+template <bool SP, bool UPDATE_APP, int MAXD>
+__device__ __forceinline__ void generic_row_reg(const ck_ldpc_code& c, float* a,
+                                                float* __restrict__ Rm, int row, bool first,
+                                                const Params& prm) {
+    RowRegs<MAXD> r;
+    row_load_cols(c, row, r);
+    row_load_q(a, Rm, first, r);
+    row_update<SP, UPDATE_APP>(a, Rm, prm, r);
+}
+
+// Two rows of one layer (row1 < 0: none), their loads in flight together.
+template <bool SP, bool UPDATE_APP, int MAXD>
+__device__ __forceinline__ void generic_rows2(const ck_ldpc_code& c, float* a,
+                                              float* __restrict__ Rm, int row0, int row1,
+                                              bool first, const Params& prm) {
+    RowRegs<MAXD> r0, r1;
+    row_load_cols(c, row0, r0);
+    row_load_cols(c, row1, r1);
+    row_load_q(a, Rm, first, r0);
+    row_load_q(a, Rm, first, r1);
+    row_update<SP, UPDATE_APP>(a, Rm, prm, r0);
+    if (row1 >= 0) row_update<SP, UPDATE_APP>(a, Rm, prm, r1);
+}
+
+template <bool SP, bool UPDATE_APP, int MAXD>
 __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, float* a,
                                             float* __restrict__ Rm, int row, bool first,
                                             const Params& prm) {
+    if (MAXD > 0) {
+        generic_row_reg<SP, UPDATE_APP, (MAXD > 0 ? MAXD : 1)>(c, a, Rm, row, first, prm);
+        return;
+    }
     const int e0 = c.row_ptr[row];
     const int e1 = c.row_ptr[row + 1];
     unsigned neg = 0u;
@@ -294,7 +442,7 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, float* a,
     }
 }
 
-template <bool SP, bool FLOOD, bool APP_LDS>
+template <bool SP, bool FLOOD, bool APP_LDS, int MAXD>
 __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
@@ -310,12 +458,12 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
     __syncthreads();
 
     int it = 0;
-    bool ok = prm.early_stop && generic_syndrome_ok(c, a);
+    bool ok = prm.early_stop && generic_syndrome_ok<MAXD>(c, a);
     while (!ok && it < prm.max_iterations) {
         const bool first = it == 0;
         if (FLOOD) {
             for (int row = threadIdx.x; row < c.num_checks; row += blockDim.x)
-                generic_row<SP, false>(c, a, Rm, row, first, prm);
+                generic_row<SP, false, MAXD>(c, a, Rm, row, first, prm);
             __syncthreads();
             for (int v = threadIdx.x; v < N; v += blockDim.x) {
                 float sum = v < c.n ? L[v] : 0.0f;
@@ -326,15 +474,22 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
         } else {
             for (int layer = 0; layer < c.num_layers; ++layer) {
                 const int end = c.layer_ptr[layer + 1];
-                for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += blockDim.x)
-                    generic_row<SP, true>(c, a, Rm, row, first, prm);
+                if (MAXD > 0 && MAXD <= 16) {
+                    const int step = static_cast<int>(blockDim.x);
+                    for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += 2 * step)
+                        generic_rows2<SP, true, (MAXD > 0 ? MAXD : 1)>(
+                            c, a, Rm, row, row + step < end ? row + step : -1, first, prm);
+                } else {
+                    for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += blockDim.x)
+                        generic_row<SP, true, MAXD>(c, a, Rm, row, first, prm);
+                }
                 __syncthreads();
             }
         }
         ++it;
-        if (prm.early_stop) ok = generic_syndrome_ok(c, a);
+        if (prm.early_stop) ok = generic_syndrome_ok<MAXD>(c, a);
     }
-    if (!prm.early_stop) ok = generic_syndrome_ok(c, a);
+    if (!prm.early_stop) ok = generic_syndrome_ok<MAXD>(c, a);
     __syncthreads();
     store_frame(c, a, frame, threadIdx.x, blockDim.x, it, ok, app_out, bits_out, iters_out,
                 valid_out);
@@ -534,16 +689,29 @@ void launch_qc_flooding(const Plan& p, const ck_ldpc_code& c, const float* llr, 
                        s, c, llr, app, bits, iters, valid, ws, R, p.F, prm);
 }
 
-template <bool SP, bool FLOOD, bool APP_LDS>
-void launch_generic(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
-                    signed char* bits, int* iters, unsigned char* valid, void* ws, long long R,
-                    const Params& prm, hipStream_t s) {
-    auto kernel = ldpc_generic<SP, FLOOD, APP_LDS>;
+template <bool SP, bool FLOOD, bool APP_LDS, int MAXD>
+void launch_generic_d(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
+                      signed char* bits, int* iters, unsigned char* valid, void* ws, long long R,
+                      const Params& prm, hipStream_t s) {
+    auto kernel = ldpc_generic<SP, FLOOD, APP_LDS, MAXD>;
     allow_lds(kernel, p.lds_bytes);
     float* msg = static_cast<float*>(ws);
     float* app_ws = APP_LDS ? nullptr : msg + static_cast<size_t>(R) * c.num_edges;
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(R)), dim3(p.threads), p.lds_bytes, s, c,
                        llr, app, bits, iters, valid, msg, app_ws, R, prm);
+}
+
+// Rows of up to 8, 16 or 32 edges take the register batch; wider rows the
+// edge-by-edge loop.
+template <bool SP, bool FLOOD, bool APP_LDS>
+void launch_generic(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
+                    signed char* bits, int* iters, unsigned char* valid, void* ws, long long R,
+                    const Params& prm, hipStream_t s) {
+    const int d = c.max_row_degree;
+    if (d <= 8) launch_generic_d<SP, FLOOD, APP_LDS, 8>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
+    else if (d <= 16) launch_generic_d<SP, FLOOD, APP_LDS, 16>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
+    else if (d <= 32) launch_generic_d<SP, FLOOD, APP_LDS, 32>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
+    else launch_generic_d<SP, FLOOD, APP_LDS, 0>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
 }
 
 }  // namespace

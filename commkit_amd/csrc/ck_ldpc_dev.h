@@ -9,7 +9,7 @@
 
 namespace ckdev {
 
-constexpr int kGenericThreads = 256;
+constexpr int kGenericThreads = 1024;
 constexpr int kMaxQcThreads = 768;  // 12 waves: up to 168 VGPRs per lane
 constexpr int kMaxFrames = 64;   // frames per QC workgroup
 constexpr int kMaxDegree = 32;   // check degree of the compressed min-sum state
