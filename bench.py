@@ -75,6 +75,54 @@ def _parse() -> argparse.Namespace:
     return ap.parse_args()
 
 
+#: The reference's own published decoder benchmark (context, not the
+#: metric's config): DVB-S2 rate 1/2 normal frames, 64 codewords, QPSK at
+#: Es/N0 1.5 dB, layered MinSum() (commkit benchmarks/bench_coding.py:37-87),
+#: median 15.745 ms on an RTX 4070 Ti.
+REFERENCE_BENCH_MS = 15.745
+
+
+def reference_benchmark(dev, reps: int = 5) -> dict:
+    """The reference's decoder benchmark, rebuilt with this library's
+    generate / apply_awgn / compute_llr (seeds 0 and 1) and timed with HIP
+    events on the launch stream: median of ``reps`` decodes."""
+    import torch
+
+    from commkit_amd import Constellation, LDPCCode, MinSum, compute_llr, decode, metrics
+    from commkit_amd.core import generate
+    from commkit_amd.impairments import apply_awgn
+
+    code = LDPCCode.dvbs2("1/2")
+    sig = generate(Constellation.qam(4), num_symbols=64 * code.n // 2, symbol_rate=1e9,
+                   code=code, rng=0, device="gpu")  # fmt: skip
+    llrs = compute_llr(apply_awgn(sig, esn0_db=1.5, rng=1), noise_var=10 ** (-1.5 / 10))
+    method = MinSum()
+    res = decode(llrs, code=code, method=method)
+    stream = torch.cuda.current_stream(dev)
+    times = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        res = decode(llrs, code=code, method=method)
+        b.record(stream)
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    ms = float(np.median(times))
+    fer = metrics.fer(res.bits, sig.reference.info_bits, info_bits=code.k)
+    return {
+        "workload": "DVB-S2 r1/2 normal (n=64800, k=32400), 64 codewords, QPSK Es/N0 1.5 dB, "
+                    "layered MinSum() (the reference's benchmarks/bench_coding.py)",
+        "ms": round(ms, 3),
+        "gbps": round(64 * code.k / (ms * 1e-3) / 1e9, 3),
+        "fer": fer,
+        "mean_iterations": float(res.iterations.float().mean().item()),
+        "reference_ms": REFERENCE_BENCH_MS,
+        "reference_hardware": "NVIDIA RTX 4070 Ti (published pytest-benchmark median)",
+        "speedup": round(REFERENCE_BENCH_MS / ms, 2),
+    }
+
+
 def copy_bandwidth(dev, nbytes: int = 1 << 30, reps: int = 5) -> float:
     """Measured device-to-device copy rate (read + write bytes per second, GB/s)."""
     import torch
@@ -763,6 +811,7 @@ def main() -> int:
         },
         "hbm_copy_GBs": round(copy_bandwidth(dev), 1),
     }
+    line["reference_benchmark"] = reference_benchmark(dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(code, c, fr.rx.cpu().numpy(), fr.info.cpu().numpy(),
                                             nv, args.cpu_seconds)  # fmt: skip
