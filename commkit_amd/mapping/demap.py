@@ -208,14 +208,18 @@ def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
     return out.reshape(*shape[:-1], shape[-1] * k) if len(shape) else out.reshape(-1)
 
 
-def llr_rows(c: "Constellation", x: Any, noise_var: Any, method: str, out: Any = None) -> Any:
+def llr_rows(c: "Constellation", x: Any, noise_var: Any, method: str, out: Any = None, *,
+             inv_s2: Any = None) -> Any:
     """LLRs of ``(R, S)`` complex device symbols, row ``r`` with noise variance ``noise_var[r]``.
 
     The rows of frames at mixed Es/N0 in one launch; row ``r`` equals
     ``llr_any(c, x[r], noise_var[r], method)`` bit for bit.  Separable
     constellations only.  ``out``: optional ``(R, S * k)`` float32 contiguous
-    device tensor to write into.  No counterpart in the reference (its
-    ``compute_llr`` takes one noise variance, ``commkit/mapping/llr.py:24-95``).
+    device tensor to write into.  ``inv_s2``: instead of ``noise_var``, the
+    ``(R,)`` float64 device tensor of ``1 / max(noise_var, 1e-20)`` (no host
+    transfer: the call can then be captured in a graph).  No counterpart in
+    the reference (its ``compute_llr`` takes one noise variance,
+    ``commkit/mapping/llr.py:24-95``).
     """
     import torch
 
@@ -225,12 +229,19 @@ def llr_rows(c: "Constellation", x: Any, noise_var: Any, method: str, out: Any =
     if x.dim() != 2 or not x.is_complex():
         raise ValueError("llr_rows: symbols must be a (rows, symbols) complex tensor")
     R, S = int(x.shape[0]), int(x.shape[1])
-    nv = np.asarray(noise_var, dtype=np.float64).reshape(-1)
-    if nv.shape[0] != R:
-        raise ValueError(f"llr_rows: {nv.shape[0]} noise variances for {R} rows")
-    if not np.all(np.isfinite(nv)) or np.any(nv < 0):
-        raise ValueError("llr_rows: noise variances must be finite and >= 0")
-    inv = torch.from_numpy(1.0 / np.maximum(nv, 1e-20)).to(x.device)
+    if inv_s2 is not None:
+        if (inv_s2.dtype != torch.float64 or tuple(inv_s2.shape) != (R,)
+                or not inv_s2.is_contiguous() or inv_s2.device != x.device):
+            raise ValueError(f"llr_rows: inv_s2 must be a contiguous ({R},) float64 tensor "
+                             "on the symbols' device")  # fmt: skip
+        inv = inv_s2
+    else:
+        nv = np.asarray(noise_var, dtype=np.float64).reshape(-1)
+        if nv.shape[0] != R:
+            raise ValueError(f"llr_rows: {nv.shape[0]} noise variances for {R} rows")
+        if not np.all(np.isfinite(nv)) or np.any(nv < 0):
+            raise ValueError("llr_rows: noise variances must be finite and >= 0")
+        inv = torch.from_numpy(1.0 / np.maximum(nv, 1e-20)).to(x.device)
     k = c.bits_per_symbol
     flat = x.to(torch.complex64).contiguous()
     if out is None:
