@@ -176,3 +176,37 @@ def test_llr_rows_equal_per_row_calls(gpu, make, method):
         llr_rows(c, xt, nv[:-1], method)
     with pytest.raises(ValueError):
         llr_rows(Constellation.psk(8), xt, nv, method)
+
+
+def test_llr_rows_inverse_variances_in_a_graph(gpu):
+    """inv_s2 on the device gives the LLRs of noise_var; a captured call
+    replays to the eager result every time, with eager work between the
+    replays."""
+    import torch
+
+    from commkit_amd.mapping.demap import llr_rows
+
+    c = Constellation.qam(64)
+    rng = np.random.default_rng(12)
+    R, S = 7, 513
+    nv = rng.uniform(0.01, 0.2, size=R)
+    x = c.points[rng.integers(0, 64, size=(R, S))] + 0.1 * rng.standard_normal((R, S))
+    xt = torch.from_numpy(x.astype(np.complex64)).cuda()
+    want = llr_rows(c, xt, nv, "maxlog")
+    inv = torch.from_numpy(1.0 / np.maximum(nv, 1e-20)).cuda()
+    out = torch.zeros_like(want)
+    assert torch.equal(llr_rows(c, xt, None, "maxlog", out=out, inv_s2=inv), want)
+    graph = torch.cuda.CUDAGraph()
+    out.zero_()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        llr_rows(c, xt, None, "maxlog", out=out, inv_s2=inv)
+    for _ in range(3):
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, want)
+        llr_rows(c, xt, nv, "maxlog")  # eager work between replays
+    with pytest.raises(ValueError, match="inv_s2"):
+        llr_rows(c, xt, None, "maxlog", inv_s2=inv.float())
