@@ -579,6 +579,7 @@ class CkLdpcCode(ctypes.Structure):
         ("qc_slot_pos", ctypes.c_void_p),
         ("qc_slot_row", ctypes.c_void_p),
         ("qc_slot_packed", ctypes.c_void_p),
+        ("slot_cols", ctypes.c_void_p),
     ]
 
 
@@ -614,16 +615,24 @@ def device_code(code: LDPCCode, device: Any) -> DeviceCode:
     info = code._encoder.info
     contiguous_info = info.size == 0 or info.size == info[-1] + 1
 
+    deg = np.diff(row_ptr)
+    width = next((w for w in (8, 16, 32) if deg.max() <= w), 0)
+
     def build() -> tuple[np.ndarray, ...]:
         base = [row_ptr, cols, layer_ptr, var_ptr, var_edges, info.astype(np.int32)]
         if qc is not None:
             base += [qc[key] for key in ("row_ptr", "slot_col", "slot_shift", "row_id",
                                          "col_ptr", "col_slot", "slot_pos", "slot_row",
                                          "slot_packed")]  # fmt: skip
+        elif width:
+            # (width, m): slot i of every schedule row, -1 past the row's end
+            slots = np.full((width, code.num_checks), -1, dtype=np.int32)
+            row_of = np.repeat(np.arange(code.num_checks), deg)
+            slots[np.arange(cols.size) - row_ptr[row_of], row_of] = cols
+            base.append(slots)
         return tuple(np.ascontiguousarray(a, dtype=np.int32) for a in base)
 
     t = device_tables(code, "decoder", device, build)
-    deg = np.diff(row_ptr)
     s = CkLdpcCode()
     s.num_checks = code.num_checks
     s.num_variables = code.num_variables
@@ -643,6 +652,8 @@ def device_code(code: LDPCCode, device: Any) -> DeviceCode:
         (s.qc_row_ptr, s.qc_slot_col, s.qc_slot_shift, s.qc_row_id, s.qc_col_ptr,
          s.qc_col_slot, s.qc_slot_pos, s.qc_slot_row,
          s.qc_slot_packed) = (x.data_ptr() for x in t[6:15])  # fmt: skip
+    elif width:
+        s.slot_cols = t[6].data_ptr()
     return DeviceCode(s, t)
 
 

@@ -11,7 +11,9 @@
 // mag = max(scale * min - offset, 0) (min2 for edge idx) or
 // phi(total - phi(|Q|)); the quasi-cyclic kernels evaluate phi from the
 // hardware exp2 / log2 (phi_fast, within the sum-product tolerance), the
-// generic kernel with logf / tanhf.  This translation unit is compiled with
+// generic kernel with logf / tanhf (with phi_fast, this compiler rejects
+// the generic kernels' flat LDS/HBM addressing).  This translation unit is
+// compiled with
 // -ffp-contract=off, so every product and sum rounds on its own, as in the
 // reference: min-sum is bit-identical to the CPU restatement, including
 // iteration counts and valid flags.  Early stop checks every row's parity of
@@ -233,12 +235,28 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
 // Generic CSR kernel
 // ---------------------------------------------------------------------------
 
+// The APP vector of the generic kernel: variables below `split` in LDS, the
+// rest in the frame's HBM workspace `glob` (ALL_LDS: all of it in LDS).  A
+// flat address serves lanes of either kind in one load instruction, so a
+// frame too long for LDS keeps its lowest-numbered variables there (for the
+// systematic codes here, the information bits: the columns of highest degree).
+template <bool ALL_LDS>
+struct GenApp {
+    float* lds;
+    float* glob;
+    int split;
+    __device__ __forceinline__ float* at(int v) const {
+        if constexpr (ALL_LDS) return lds + v;
+        return v < split ? lds + v : glob + v;
+    }
+};
+
 // MAXD > 0: rows of at most MAXD edges, whose columns and values are loaded
 // as one batch into registers (a row then costs one memory round trip for
 // its columns and one for its values, instead of two per edge); MAXD = 0:
 // any degree, edge by edge.
-template <int MAXD>
-__device__ bool generic_syndrome_ok(const ck_ldpc_code& c, const float* a) {
+template <int MAXD, class App>
+__device__ bool generic_syndrome_ok(const ck_ldpc_code& c, const App& A) {
     // Rounds of blockDim rows; the check stops at the first round with an
     // unsatisfied row (the frame is not a codeword), which while decoding
     // is usually the first.
@@ -257,11 +275,11 @@ __device__ bool generic_syndrome_ok(const ck_ldpc_code& c, const float* a) {
 #pragma unroll
                 for (int i = 0; i < D; ++i) col[i] = c.cols[e0 + i < e1 ? e0 + i : last];
 #pragma unroll
-                for (int i = 0; i < D; ++i) v[i] = a[col[i]];
+                for (int i = 0; i < D; ++i) v[i] = *A.at(col[i]);
 #pragma unroll
                 for (int i = 0; i < D; ++i) parity ^= (e0 + i < e1 && v[i] < 0.0f) ? 1 : 0;
             } else {
-                for (int e = e0; e < e1; ++e) parity ^= a[c.cols[e]] < 0.0f;
+                for (int e = e0; e < e1; ++e) parity ^= *A.at(c.cols[e]) < 0.0f;
             }
         }
         if (__syncthreads_or(parity)) return false;
@@ -274,9 +292,14 @@ __device__ bool generic_syndrome_ok(const ck_ldpc_code& c, const float* a) {
 // read once; within a layer no other row touches the row's columns, so the
 // second pass's APP equals the first's).  Loading and updating are split so
 // that a lane can put the loads of two rows in flight before either update.
+// Messages of slot i of the row at Rm[base + i * stride]: the row's CSR edges
+// (base = first edge, stride 1; flooding, whose variable update reads them by
+// edge id), or slot-major (base = schedule row, stride = m; layered), where
+// the lanes of a wave, on consecutive rows, touch consecutive words.
 template <int MAXD>
 struct RowRegs {
-    int e0;
+    int base;
+    int stride;
     int col[MAXD];
     float q[MAXD];
 };
@@ -285,11 +308,22 @@ struct RowRegs {
 // last edge, or edge / variable 0 for no row), and the padding slots are
 // marked afterwards: loads under per-slot conditions would each be waited
 // for before the next is issued.
-template <int MAXD>
+template <int MAXD, bool SLOT_MAJOR>
 __device__ __forceinline__ void row_load_cols(const ck_ldpc_code& c, int row, RowRegs<MAXD>& r) {
+    if (SLOT_MAJOR && c.slot_cols != nullptr) {  // one coalesced load per slot
+        r.base = row >= 0 ? row : 0;
+        r.stride = c.num_checks;
+#pragma unroll
+        for (int i = 0; i < MAXD; ++i) {
+            const int v = c.slot_cols[i * c.num_checks + r.base];
+            r.col[i] = row >= 0 ? v : -1;
+        }
+        return;
+    }
     const int e0 = row >= 0 ? c.row_ptr[row] : 0;
     const int e1 = row >= 0 ? c.row_ptr[row + 1] : 0;
-    r.e0 = e0;
+    r.base = SLOT_MAJOR ? (row >= 0 ? row : 0) : e0;
+    r.stride = SLOT_MAJOR ? c.num_checks : 1;
     const int last = e1 > e0 ? e1 - 1 : 0;
 #pragma unroll
     for (int i = 0; i < MAXD; ++i) {
@@ -298,23 +332,23 @@ __device__ __forceinline__ void row_load_cols(const ck_ldpc_code& c, int row, Ro
     }
 }
 
-template <int MAXD>
-__device__ __forceinline__ void row_load_q(const float* a, const float* __restrict__ Rm,
+template <int MAXD, class App>
+__device__ __forceinline__ void row_load_q(const App& A, const float* __restrict__ Rm,
                                            bool first, RowRegs<MAXD>& r) {
     float av[MAXD];
     float rv[MAXD];
 #pragma unroll
     for (int i = 0; i < MAXD; ++i) {
         const bool in = r.col[i] >= 0;
-        av[i] = a[in ? r.col[i] : 0];
-        rv[i] = first ? 0.0f : Rm[in ? r.e0 + i : r.e0];
+        av[i] = *A.at(in ? r.col[i] : 0);
+        rv[i] = first ? 0.0f : Rm[in ? r.base + i * r.stride : r.base];
     }
 #pragma unroll
     for (int i = 0; i < MAXD; ++i) r.q[i] = av[i] - rv[i];
 }
 
-template <bool SP, bool UPDATE_APP, int MAXD>
-__device__ __forceinline__ void row_update(float* a, float* __restrict__ Rm, const Params& prm,
+template <bool SP, bool UPDATE_APP, int MAXD, class App>
+__device__ __forceinline__ void row_update(const App& A, float* __restrict__ Rm, const Params& prm,
                                            const RowRegs<MAXD>& r) {
     unsigned neg = 0u;
     if (SP) {
@@ -330,8 +364,8 @@ __device__ __forceinline__ void row_update(float* a, float* __restrict__ Rm, con
             if (r.col[i] >= 0) {
                 const float mag = phi(total - phi(fabsf(r.q[i])));
                 const float rn = (neg ^ ((r.q[i] < 0.0f) ? 1u : 0u)) ? -mag : mag;
-                if (UPDATE_APP) a[r.col[i]] = r.q[i] + rn;
-                Rm[r.e0 + i] = rn;
+                if (UPDATE_APP) *A.at(r.col[i]) = r.q[i] + rn;
+                Rm[r.base + i * r.stride] = rn;
             }
         return;
     }
@@ -358,41 +392,42 @@ __device__ __forceinline__ void row_update(float* a, float* __restrict__ Rm, con
         if (r.col[i] >= 0) {
             const float mag = (i == idx) ? m2 : m1;
             const float rn = (neg ^ ((r.q[i] < 0.0f) ? 1u : 0u)) ? -mag : mag;
-            if (UPDATE_APP) a[r.col[i]] = r.q[i] + rn;
-            Rm[r.e0 + i] = rn;
+            if (UPDATE_APP) *A.at(r.col[i]) = r.q[i] + rn;
+            Rm[r.base + i * r.stride] = rn;
         }
 }
 
-template <bool SP, bool UPDATE_APP, int MAXD>
-__device__ __forceinline__ void generic_row_reg(const ck_ldpc_code& c, float* a,
+template <bool SP, bool UPDATE_APP, int MAXD, bool SLOT_MAJOR, class App>
+__device__ __forceinline__ void generic_row_reg(const ck_ldpc_code& c, const App& A,
                                                 float* __restrict__ Rm, int row, bool first,
                                                 const Params& prm) {
     RowRegs<MAXD> r;
-    row_load_cols(c, row, r);
-    row_load_q(a, Rm, first, r);
-    row_update<SP, UPDATE_APP>(a, Rm, prm, r);
+    row_load_cols<MAXD, SLOT_MAJOR>(c, row, r);
+    row_load_q(A, Rm, first, r);
+    row_update<SP, UPDATE_APP>(A, Rm, prm, r);
 }
 
 // Two rows of one layer (row1 < 0: none), their loads in flight together.
-template <bool SP, bool UPDATE_APP, int MAXD>
-__device__ __forceinline__ void generic_rows2(const ck_ldpc_code& c, float* a,
+template <bool SP, bool UPDATE_APP, int MAXD, bool SLOT_MAJOR, class App>
+__device__ __forceinline__ void generic_rows2(const ck_ldpc_code& c, const App& A,
                                               float* __restrict__ Rm, int row0, int row1,
                                               bool first, const Params& prm) {
     RowRegs<MAXD> r0, r1;
-    row_load_cols(c, row0, r0);
-    row_load_cols(c, row1, r1);
-    row_load_q(a, Rm, first, r0);
-    row_load_q(a, Rm, first, r1);
-    row_update<SP, UPDATE_APP>(a, Rm, prm, r0);
-    if (row1 >= 0) row_update<SP, UPDATE_APP>(a, Rm, prm, r1);
+    row_load_cols<MAXD, SLOT_MAJOR>(c, row0, r0);
+    row_load_cols<MAXD, SLOT_MAJOR>(c, row1, r1);
+    row_load_q(A, Rm, first, r0);
+    row_load_q(A, Rm, first, r1);
+    row_update<SP, UPDATE_APP>(A, Rm, prm, r0);
+    if (row1 >= 0) row_update<SP, UPDATE_APP>(A, Rm, prm, r1);
 }
 
-template <bool SP, bool UPDATE_APP, int MAXD>
-__device__ __forceinline__ void generic_row(const ck_ldpc_code& c, float* a,
+template <bool SP, bool UPDATE_APP, int MAXD, bool SLOT_MAJOR, class App>
+__device__ __forceinline__ void generic_row(const ck_ldpc_code& c, const App& A,
                                             float* __restrict__ Rm, int row, bool first,
                                             const Params& prm) {
     if (MAXD > 0) {
-        generic_row_reg<SP, UPDATE_APP, (MAXD > 0 ? MAXD : 1)>(c, a, Rm, row, first, prm);
+        generic_row_reg<SP, UPDATE_APP, (MAXD > 0 ? MAXD : 1), SLOT_MAJOR>(c, A, Rm, row, first,
+                                                                           prm);
         return;
     }
     const int e0 = c.row_ptr[row];
@@ -401,16 +436,16 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, float* a,
     if (SP) {
         float total = 0.0f;
         for (int e = e0; e < e1; ++e) {
-            const float q = a[c.cols[e]] - (first ? 0.0f : Rm[e]);
+            const float q = *A.at(c.cols[e]) - (first ? 0.0f : Rm[e]);
             neg ^= (q < 0.0f) ? 1u : 0u;
             total = total + phi(fabsf(q));
         }
         for (int e = e0; e < e1; ++e) {
             const int col = c.cols[e];
-            const float q = a[col] - (first ? 0.0f : Rm[e]);
+            const float q = *A.at(col) - (first ? 0.0f : Rm[e]);
             const float mag = phi(total - phi(fabsf(q)));
             const float rn = (neg ^ ((q < 0.0f) ? 1u : 0u)) ? -mag : mag;
-            if (UPDATE_APP) a[col] = q + rn;
+            if (UPDATE_APP) *A.at(col) = q + rn;
             Rm[e] = rn;
         }
         return;
@@ -419,7 +454,7 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, float* a,
     float min2 = ck::kInf;
     int idx = -1;
     for (int e = e0; e < e1; ++e) {
-        const float q = a[c.cols[e]] - (first ? 0.0f : Rm[e]);
+        const float q = *A.at(c.cols[e]) - (first ? 0.0f : Rm[e]);
         neg ^= (q < 0.0f) ? 1u : 0u;
         const float mag = fabsf(q);
         if (mag < min1) {
@@ -434,10 +469,10 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, float* a,
     const float m2 = fmaxf(prm.scale * min2 - prm.offset, 0.0f);
     for (int e = e0; e < e1; ++e) {
         const int col = c.cols[e];
-        const float q = a[col] - (first ? 0.0f : Rm[e]);
+        const float q = *A.at(col) - (first ? 0.0f : Rm[e]);
         const float mag = (e == idx) ? m2 : m1;
         const float rn = (neg ^ ((q < 0.0f) ? 1u : 0u)) ? -mag : mag;
-        if (UPDATE_APP) a[col] = q + rn;
+        if (UPDATE_APP) *A.at(col) = q + rn;
         Rm[e] = rn;
     }
 }
@@ -447,28 +482,29 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
     unsigned char* __restrict__ valid_out, float* __restrict__ msg_ws,
-    float* __restrict__ app_ws, long long R, Params prm) {
+    float* __restrict__ app_ws, long long R, Params prm, int lds_vars, long long msg_stride) {
     extern __shared__ float lds[];
     const long long frame = blockIdx.x;
     const int N = c.num_variables;
-    float* a = APP_LDS ? lds : app_ws + static_cast<size_t>(frame) * N;
-    float* Rm = msg_ws + static_cast<size_t>(frame) * c.num_edges;
+    float* glob = APP_LDS ? nullptr : app_ws + static_cast<size_t>(frame) * N;
+    const GenApp<APP_LDS> A{lds, glob, APP_LDS ? N : lds_vars};
+    float* Rm = msg_ws + static_cast<size_t>(frame) * msg_stride;
     const float* L = llr + frame * c.n;
-    for (int v = threadIdx.x; v < N; v += blockDim.x) a[v] = v < c.n ? L[v] : 0.0f;
+    for (int v = threadIdx.x; v < N; v += blockDim.x) *A.at(v) = v < c.n ? L[v] : 0.0f;
     __syncthreads();
 
     int it = 0;
-    bool ok = prm.early_stop && generic_syndrome_ok<MAXD>(c, a);
+    bool ok = prm.early_stop && generic_syndrome_ok<MAXD>(c, A);
     while (!ok && it < prm.max_iterations) {
         const bool first = it == 0;
         if (FLOOD) {
             for (int row = threadIdx.x; row < c.num_checks; row += blockDim.x)
-                generic_row<SP, false, MAXD>(c, a, Rm, row, first, prm);
+                generic_row<SP, false, MAXD, false>(c, A, Rm, row, first, prm);
             __syncthreads();
             for (int v = threadIdx.x; v < N; v += blockDim.x) {
                 float sum = v < c.n ? L[v] : 0.0f;
                 for (int t = c.var_ptr[v]; t < c.var_ptr[v + 1]; ++t) sum = sum + Rm[c.var_edges[t]];
-                a[v] = sum;
+                *A.at(v) = sum;
             }
             __syncthreads();
         } else {
@@ -477,22 +513,25 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
                 if (MAXD > 0 && MAXD <= 16) {
                     const int step = static_cast<int>(blockDim.x);
                     for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += 2 * step)
-                        generic_rows2<SP, true, (MAXD > 0 ? MAXD : 1)>(
-                            c, a, Rm, row, row + step < end ? row + step : -1, first, prm);
+                        generic_rows2<SP, true, (MAXD > 0 ? MAXD : 1), true>(
+                            c, A, Rm, row, row + step < end ? row + step : -1, first, prm);
                 } else {
                     for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += blockDim.x)
-                        generic_row<SP, true, MAXD>(c, a, Rm, row, first, prm);
+                        generic_row<SP, true, MAXD, (MAXD > 0)>(c, A, Rm, row, first, prm);
                 }
                 __syncthreads();
             }
         }
         ++it;
-        if (prm.early_stop) ok = generic_syndrome_ok<MAXD>(c, a);
+        if (prm.early_stop) ok = generic_syndrome_ok<MAXD>(c, A);
     }
-    if (!prm.early_stop) ok = generic_syndrome_ok<MAXD>(c, a);
+    if (!prm.early_stop) ok = generic_syndrome_ok<MAXD>(c, A);
+    if (!APP_LDS) {  // the LDS part back to the workspace, read from there
+        for (int v = threadIdx.x; v < A.split; v += blockDim.x) glob[v] = lds[v];
+    }
     __syncthreads();
-    store_frame(c, a, frame, threadIdx.x, blockDim.x, it, ok, app_out, bits_out, iters_out,
-                valid_out);
+    store_frame(c, APP_LDS ? lds : glob, frame, threadIdx.x, blockDim.x, it, ok, app_out,
+                bits_out, iters_out, valid_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -512,6 +551,7 @@ struct Plan {
     int threads;
     size_t lds_bytes;
     size_t ws_per_frame;  // bytes
+    size_t msg_floats;    // generic: messages per frame
 };
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -570,6 +610,19 @@ namespace {
 
 using ck::check_code;
 using ck::check_method;
+
+// Register batch width of the generic kernel for rows of up to d edges (0:
+// edge by edge), the same rule launch_generic dispatches on.
+int generic_maxd(int d) { return d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : 0; }
+
+// Messages per frame of the generic kernel: slot-major (rows x the batch
+// width) for the layered schedule with register batches, one per CSR edge
+// otherwise.
+size_t generic_msg_floats(const ck_ldpc_code& c, bool flood) {
+    const int w = generic_maxd(c.max_row_degree);
+    return (!flood && w > 0) ? static_cast<size_t>(w) * c.num_checks
+                             : static_cast<size_t>(c.num_edges);
+}
 
 int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, Plan* plan) {
     if (int32_t st = check_code(c)) return st;
@@ -650,8 +703,10 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, P
         p.kind = 1;
         p.app_lds = app_bytes <= lds_max;
         p.threads = kGenericThreads;
-        p.lds_bytes = p.app_lds ? app_bytes : 0;
-        p.ws_per_frame = static_cast<size_t>(c->num_edges) * 4 + (p.app_lds ? 0 : app_bytes);
+        // too long for LDS: its first lds_max / 4 variables there, the rest in HBM
+        p.lds_bytes = p.app_lds ? app_bytes : (lds_max & ~static_cast<size_t>(255));
+        p.msg_floats = generic_msg_floats(*c, p.flood);
+        p.ws_per_frame = p.msg_floats * 4 + (p.app_lds ? 0 : app_bytes);
     }
     *plan = p;
     return CK_OK;
@@ -696,9 +751,10 @@ void launch_generic_d(const Plan& p, const ck_ldpc_code& c, const float* llr, fl
     auto kernel = ldpc_generic<SP, FLOOD, APP_LDS, MAXD>;
     allow_lds(kernel, p.lds_bytes);
     float* msg = static_cast<float*>(ws);
-    float* app_ws = APP_LDS ? nullptr : msg + static_cast<size_t>(R) * c.num_edges;
+    float* app_ws = APP_LDS ? nullptr : msg + static_cast<size_t>(R) * p.msg_floats;
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(R)), dim3(p.threads), p.lds_bytes, s, c,
-                       llr, app, bits, iters, valid, msg, app_ws, R, prm);
+                       llr, app, bits, iters, valid, msg, app_ws, R, prm,
+                       static_cast<int>(p.lds_bytes / 4), static_cast<long long>(p.msg_floats));
 }
 
 // Rows of up to 8, 16 or 32 edges take the register batch; wider rows the
@@ -707,10 +763,10 @@ template <bool SP, bool FLOOD, bool APP_LDS>
 void launch_generic(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
                     signed char* bits, int* iters, unsigned char* valid, void* ws, long long R,
                     const Params& prm, hipStream_t s) {
-    const int d = c.max_row_degree;
-    if (d <= 8) launch_generic_d<SP, FLOOD, APP_LDS, 8>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
-    else if (d <= 16) launch_generic_d<SP, FLOOD, APP_LDS, 16>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
-    else if (d <= 32) launch_generic_d<SP, FLOOD, APP_LDS, 32>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
+    const int d = generic_maxd(c.max_row_degree);
+    if (d == 8) launch_generic_d<SP, FLOOD, APP_LDS, 8>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
+    else if (d == 16) launch_generic_d<SP, FLOOD, APP_LDS, 16>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
+    else if (d == 32) launch_generic_d<SP, FLOOD, APP_LDS, 32>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
     else launch_generic_d<SP, FLOOD, APP_LDS, 0>(p, c, llr, app, bits, iters, valid, ws, R, prm, s);
 }
 

@@ -149,6 +149,11 @@ typedef struct ck_ldpc_code {
     const int32_t* qc_slot_row;   /* (S) scheduled block row of each slot    */
     const int32_t* qc_slot_packed;/* (S + 32, 2) {block column * Z, shift}
                                      per slot, then 32 zero entries         */
+    const int32_t* slot_cols;     /* (w, m) slot-major columns of the rows in
+                                     schedule order, -1 past a row's end,
+                                     w = 8, 16 or 32 (the smallest at least
+                                     max_row_degree); NULL: the layered
+                                     generic kernel reads the CSR            */
 } ck_ldpc_code;
 
 /* Decoding method.  rule/scale/offset/max_iterations/early_stop follow
