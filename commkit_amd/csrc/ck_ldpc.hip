@@ -407,18 +407,20 @@ __device__ __forceinline__ void generic_row_reg(const ck_ldpc_code& c, const App
     row_update<SP, UPDATE_APP>(A, Rm, prm, r);
 }
 
-// Two rows of one layer (row1 < 0: none), their loads in flight together.
-template <bool SP, bool UPDATE_APP, int MAXD, bool SLOT_MAJOR, class App>
-__device__ __forceinline__ void generic_rows2(const ck_ldpc_code& c, const App& A,
-                                              float* __restrict__ Rm, int row0, int row1,
-                                              bool first, const Params& prm) {
-    RowRegs<MAXD> r0, r1;
-    row_load_cols<MAXD, SLOT_MAJOR>(c, row0, r0);
-    row_load_cols<MAXD, SLOT_MAJOR>(c, row1, r1);
-    row_load_q(A, Rm, first, r0);
-    row_load_q(A, Rm, first, r1);
-    row_update<SP, UPDATE_APP>(A, Rm, prm, r0);
-    if (row1 >= 0) row_update<SP, UPDATE_APP>(A, Rm, prm, r1);
+// NR rows of one layer (a negative row: none), all their loads in flight
+// before any update.
+template <bool SP, bool UPDATE_APP, int MAXD, bool SLOT_MAJOR, int NR, class App>
+__device__ __forceinline__ void generic_rows(const ck_ldpc_code& c, const App& A,
+                                             float* __restrict__ Rm, const int (&rows)[NR],
+                                             bool first, const Params& prm) {
+    RowRegs<MAXD> r[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) row_load_cols<MAXD, SLOT_MAJOR>(c, rows[j], r[j]);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) row_load_q(A, Rm, first, r[j]);
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+        if (rows[j] >= 0) row_update<SP, UPDATE_APP>(A, Rm, prm, r[j]);
 }
 
 template <bool SP, bool UPDATE_APP, int MAXD, bool SLOT_MAJOR, class App>
@@ -511,10 +513,17 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
             for (int layer = 0; layer < c.num_layers; ++layer) {
                 const int end = c.layer_ptr[layer + 1];
                 if (MAXD > 0 && MAXD <= 16) {
+                    // two rows to a lane (four measured slower: 2.3 against
+                    // 2.0 ms on DVB-S2 rate 1/2, 64 frames)
+                    constexpr int NR = 2;
                     const int step = static_cast<int>(blockDim.x);
-                    for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += 2 * step)
-                        generic_rows2<SP, true, (MAXD > 0 ? MAXD : 1), true>(
-                            c, A, Rm, row, row + step < end ? row + step : -1, first, prm);
+                    for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += NR * step) {
+                        int rows[NR];
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) rows[j] = row + j * step < end ? row + j * step : -1;
+                        generic_rows<SP, true, (MAXD > 0 ? MAXD : 1), true, NR>(c, A, Rm, rows,
+                                                                                first, prm);
+                    }
                 } else {
                     for (int row = c.layer_ptr[layer] + threadIdx.x; row < end; row += blockDim.x)
                         generic_row<SP, true, MAXD, (MAXD > 0)>(c, A, Rm, row, first, prm);
