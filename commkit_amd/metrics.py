@@ -88,18 +88,20 @@ def _skip(x: Any, n: int, name: str) -> Any:
     return x
 
 
-def _errors(rx: Any, tx: Any, block: int) -> tuple[np.ndarray, np.ndarray]:
-    """Bit errors and blocks in error per row, from ``ck_bit_errors``."""
+def _errors(rx: Any, tx: Any, block: int | None) -> tuple[np.ndarray, np.ndarray | None]:
+    """Bit errors and blocks of ``block`` bits in error per row, from
+    ``ck_bit_errors`` (``block`` None: bit errors only)."""
     import torch
 
     rows, n = int(rx.shape[0]), int(rx.shape[1])
     bit_err = torch.empty(rows, dtype=torch.int64, device=rx.device)
-    block_err = torch.empty(rows, dtype=torch.int64, device=rx.device)
+    block_err = None if block is None else torch.empty(rows, dtype=torch.int64, device=rx.device)
     with torch.cuda.device(rx.device):
         _native.check(_native.lib().ck_bit_errors(
-            rx.data_ptr(), tx.data_ptr(), rows, n, block, bit_err.data_ptr(),
-            block_err.data_ptr(), _native.stream_of(rx.device)))  # fmt: skip
-    return bit_err.cpu().numpy(), block_err.cpu().numpy()
+            rx.data_ptr(), tx.data_ptr(), rows, n, n if block is None else block,
+            bit_err.data_ptr(), None if block_err is None else block_err.data_ptr(),
+            _native.stream_of(rx.device)))  # fmt: skip
+    return bit_err.cpu().numpy(), None if block_err is None else block_err.cpu().numpy()
 
 
 def _hard_bits(signal: Signal, constellation: Any, name: str) -> tuple[Any, Any]:
@@ -159,7 +161,7 @@ def ber(bits: Any, reference: Any = None, *, constellation: Any = None,
     n = int(rx2.shape[1])
     if n == 0:
         raise ValueError(f"{name}: nothing to measure (no bits).")
-    errors, _ = _errors(rx2, tx2, n)
+    errors, _ = _errors(rx2, tx2, None)
     return _report(errors / n, one_d)
 
 

@@ -364,7 +364,8 @@ int32_t ck_ccdm_dematch(const int32_t* symbols, int64_t rows, int64_t n,
 
 /* Per row of `bits` and `reference` (rows, n) int8 0/1: bit_errors[r] = the
  * differing bits, block_errors[r] = the blocks of `block` bits (block
- * divides n) with any: ber and fer (commkit/metrics/error_rates.py:30-181). */
+ * divides n) with any: ber and fer (commkit/metrics/error_rates.py:30-181).
+ * block_errors may be NULL (bit counts only). */
 int32_t ck_bit_errors(const int8_t* bits, const int8_t* reference, int64_t rows,
                       int64_t n, int64_t block, int64_t* bit_errors,
                       int64_t* block_errors, void* stream);

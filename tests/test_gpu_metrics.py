@@ -200,3 +200,16 @@ def test_archive_loads_onto_the_gpu(tmp_path):
     llrs = compute_llr(back, noise_var=float(fr.noise_var[0]))
     res = decode(llrs, code=back.reference.code)
     np.testing.assert_array_equal(fer(res, back), np.zeros(4))
+
+
+def test_ber_of_a_long_row():
+    """One row of 5,000,011 bits: ber splits it over spans of 65,536 bits
+    (the last one short) and still counts every error."""
+    import torch
+
+    rng = np.random.default_rng(8)
+    tx = rng.integers(0, 2, size=5_000_011, dtype=np.int8)
+    flips = rng.random(tx.size) < 1e-3
+    rx = tx ^ flips.astype(np.int8)
+    assert ber(torch.from_numpy(rx).to(_dev()), tx) == flips.sum() / tx.size
+    assert ber(rx[3:], tx[3:]) == flips[3:].sum() / (tx.size - 3)  # unaligned start
