@@ -440,18 +440,29 @@ __global__ __launch_bounds__(kLanes) void ccdm_dematch_group(
     uint64_t hi = 0;  // limbs base - 2, base - 1
     uint64_t lo = 0;  // limbs base, base + 1
     long long base = -1;
-    for (long long t = 0; t < n; ++t) {
-        const int s = sym[t];
-        const bool in_range = s >= 0 && s < M;
-        const int cs = group_max<G>((in_range && j == s) ? cnt : 0);
-        if (!in_range || cs == 0) {
+    // One step of the decoder per received point; false: stop (invalid).
+    auto step = [&](const int s) -> bool {
+        // The received point is known, so instead of every point's start the
+        // group needs three sums of the lanes' shares (all of them, those
+        // below s, that of s) and the count of s; they are independent, so
+        // their moves overlap.  The remainder of the split goes to the
+        // largest count, lowest index among equals (`big`), as in the coder.
+        const double inv = 1.0 / static_cast<double>(remaining);
+        const int64_t sh = share_of_point(static_cast<uint64_t>(rng), static_cast<uint32_t>(cnt),
+                                          static_cast<uint32_t>(remaining), inv);
+        const int64_t total = group_sum<G>(sh);
+        const int64_t below = group_sum<G>(j < s ? sh : 0);
+        const int64_t mine = group_sum<G>(j == s ? sh : 0);
+        const int cs = group_max<G>(j == s ? cnt : 0);
+        if (!(s >= 0 && s < M) || cs == 0) {  // out of range, or no such point left
             ok = false;
-            break;
+            return false;
         }
-        int64_t sh;
-        int64_t start;
-        group_split<G>(cnt, j, rng, remaining, &sh, &start);
-        const int64_t cum = group_sum<G>(j == s ? start : 0);
+        const int cmax = group_max<G>(cnt);
+        const int big = group_min<G>(cnt == cmax ? j : G);
+        const int64_t rest = rng - total;
+        const int64_t cum = below + (big < s ? rest : 0);
+        const int64_t sh_s = mine + (big == s ? rest : 0);
         const long long i = (shift - 1) >> 5;
         while (base < i - 1) {
             if (j == 0) emit(low, R, limbs, base - 2, hi >> 32);
@@ -471,16 +482,38 @@ __global__ __launch_bounds__(kLanes) void ccdm_dematch_group(
                 }
             }
         }
-        rng = group_sum<G>(j == s ? sh : 0);
+        rng = sh_s;
         cnt -= (j == s) ? 1 : 0;
         --remaining;
-        for (int g = 0; g < 32 && rng < kHalf; ++g) {  // rng >= 1: at most 31
-            rng <<= 1;
-            ++shift;
+        {  // renormalize to [2^31, 2^32) in one shift (as the bounded doubling loop)
+            const int z = rng > 0 ? __builtin_clzll(static_cast<uint64_t>(rng)) - 32 : 32;
+            const int d = z < 0 ? 0 : (z > 32 ? 32 : z);
+            rng <<= d;
+            shift += d;
         }
         if (shift > buffer_bits) {
             ok = false;
-            break;
+            return false;
+        }
+        return true;
+    };
+    // The points are loaded kAhead steps before their step, into a ring of
+    // registers (static indices: the ring is walked by an unrolled loop), so
+    // a step never waits for memory.
+    constexpr int kAhead = 8;
+    int ring[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) ring[u] = u < n ? sym[u] : 0;
+    bool go = true;
+    for (long long t0 = 0; go && t0 < n; t0 += kAhead) {
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const long long t = t0 + u;
+            if (go && t < n) {
+                const int s = ring[u];
+                ring[u] = t + kAhead < n ? sym[t + kAhead] : 0;
+                go = step(s);
+            }
         }
     }
     if (j == 0) {
