@@ -394,7 +394,7 @@ def run_shaped(args: argparse.Namespace) -> dict:
 
     from commkit_amd import Constellation, SumProduct
     from commkit_amd.coding.ldpc import _decode
-    from commkit_amd.mapping.demap import llr_any
+    from commkit_amd.mapping.demap import llr_permuted
     from commkit_amd.operating import SHAPED_WATERFALL_DB
     from commkit_amd.workloads import make_pas_frames, seeded_code
 
@@ -422,9 +422,12 @@ def run_shaped(args: argparse.Namespace) -> dict:
     pas = fr.pas
     ccdm = pas.matcher
 
+    perm32 = fr.perm.to(torch.int32).contiguous()
+
     def demap():
-        llr = llr_any(c, rx, nv, "exact")
-        return llr[: F * code.n].view(F, code.n)[:, fr.iperm].contiguous()
+        # exact LLRs with the prior, written straight into the codewords' bit
+        # layout (the filler symbols after the last codeword carry none)
+        return llr_permuted(c, rx, nv, "exact", perm32, F)
 
     def dematch(fec_bits):
         payload, ok = ccdm._dematch_labels(fec_bits[:, : ccdm.n])
@@ -494,7 +497,7 @@ def run_shaped(args: argparse.Namespace) -> dict:
         "post_fec_ber": fec_errors / (F * code.k),
         "fer": float((errors > 0).mean()),
         "mean_iterations": float(iters_h.mean()),
-        "kernels_ms": {"demap_and_gather": round(demap_ms, 4), "decode": round(decode_ms, 4)},
+        "kernels_ms": {"demap": round(demap_ms, 4), "decode": round(decode_ms, 4)},
         "pas": {
             "payload_bits_per_codeword": pas.k,
             "dematch_ms": round(dematch_ms, 4),
@@ -518,7 +521,8 @@ def run_shaped(args: argparse.Namespace) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "bytes_per_launch": demap_bytes,
-            "note": "demap_and_gather includes the gather of codeword LLRs",
+            "note": "the demapper writes the LLRs straight into the codewords' bit layout "
+                    "(ck_llr_separable_permuted)",
         },
         "served": served,
     }
@@ -535,7 +539,7 @@ def shaped_served(args, code, c, fr, nv: float, F: int, method, dev) -> dict:
     lane per codeword)."""
     import torch
 
-    from commkit_amd.mapping.demap import llr_any
+    from commkit_amd.mapping.demap import llr_permuted
     from commkit_amd.serve import Schedule, Server, drive
 
     rng = np.random.Generator(np.random.PCG64(11))
@@ -549,12 +553,11 @@ def shaped_served(args, code, c, fr, nv: float, F: int, method, dev) -> dict:
                      np.zeros(n_req, np.int64))  # fmt: skip
     S = code.n // c.bits_per_symbol
     rx = fr.rx.reshape(-1)
-    iperm = fr.iperm.to(dev)
+    perm32 = fr.perm.to(device=dev, dtype=torch.int32).contiguous()
 
     def fill(slot, i):
         a, f = int(first[i]), int(sizes[i])
-        llr = llr_any(c, rx[a * S:(a + f) * S], nv, "exact").view(f, code.n)
-        torch.index_select(llr, 1, iperm, out=slot.llrs)
+        llr_permuted(c, rx[a * S:(a + f) * S], nv, "exact", perm32, f, out=slot.llrs)
 
     # Results are copied out of the ring as requests complete (the slot is
     # reused after release), then scored once at the end.

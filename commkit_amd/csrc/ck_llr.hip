@@ -61,6 +61,32 @@ __device__ __forceinline__ void store_tile(const float* tile, float* __restrict_
     }
 }
 
+// Stores `count` staged symbols like store_tile, but the LLR of flat index t
+// goes to out[(t / period) * period + perm[t % period]] and indices from
+// `limit` on are dropped: the bit layout of a code's blocks (probabilistic
+// amplitude shaping) applied on the way out.  A tile may span several blocks.
+__device__ __forceinline__ void store_tile_perm(const float* tile, float* __restrict__ out,
+                                                long long base, int count, int k,
+                                                const int* __restrict__ perm, long long period,
+                                                long long limit) {
+    const int stride = tile_stride(k);
+    const long long t0 = base * k;
+    const long long blk0 = t0 / period;
+    const long long off0 = t0 - blk0 * period;
+    const int total = count * k;
+    for (int j = threadIdx.x; j < total; j += kThreads) {
+        if (t0 + j >= limit) break;  // j grows: every later index is past it too
+        long long off = off0 + j;
+        long long blk = blk0;
+        while (off >= period) {
+            off -= period;
+            ++blk;
+        }
+        const int sym = j / k;
+        out[blk * period + perm[off]] = tile[sym * stride + (j - sym * k)];
+    }
+}
+
 // LDS copy of one axis with 2^KB levels: float64 levels and log prior for
 // the metric differences, float32 copies for locating maxima.
 template <int KB>
@@ -270,7 +296,7 @@ template <int KB, bool CLOSED>
 __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
     const float2* __restrict__ x, long long n, ck_axis re, ck_axis im, float inv_s2f,
     double inv_s2, int exact, float* __restrict__ out, const double* __restrict__ inv_rows,
-    long long row_len) {
+    long long row_len, const int* __restrict__ perm, long long period, long long limit) {
     constexpr int K = 2 * KB;
     __shared__ AxisTab<KB> tab[2];
     __shared__ float tile[kThreads * (K | 1)];
@@ -303,7 +329,11 @@ __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
         }
         __syncthreads();
         const long long left = n - base;
-        store_tile(tile, out, base, left < kThreads ? static_cast<int>(left) : kThreads, K);
+        const int count = left < kThreads ? static_cast<int>(left) : kThreads;
+        if (perm != nullptr)
+            store_tile_perm(tile, out, base, count, K, perm, period, limit);
+        else
+            store_tile(tile, out, base, count, K);
         __syncthreads();
     }
 }
@@ -485,7 +515,8 @@ int32_t check_axis(const ck_axis* a, const char* name, bool closed) {
 template <bool CLOSED>
 int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_axis* im,
                          double inv_s2, int exact, float* out, void* stream,
-                         const double* inv_rows = nullptr, int64_t row_len = 1) {
+                         const double* inv_rows = nullptr, int64_t row_len = 1,
+                         const int32_t* perm = nullptr, int64_t period = 1, int64_t limit = 0) {
     CK_REQUIRE(n >= 0, "n = %lld < 0", static_cast<long long>(n));
     if (int32_t st = check_axis(re, "re", CLOSED)) return st;
     if (int32_t st = check_axis(im, "im", CLOSED)) return st;
@@ -504,7 +535,8 @@ int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_a
         hipLaunchKernelGGL((llr_separable_kernel<KB, CLOSED>), dim3(grid), dim3(kThreads), 0, \
                            s, xs, static_cast<long long>(n), *re, *im,                        \
                            static_cast<float>(inv_s2), inv_s2, exact, out, inv_rows,          \
-                           static_cast<long long>(row_len));                                  \
+                           static_cast<long long>(row_len), perm,                             \
+                           static_cast<long long>(period), static_cast<long long>(limit));    \
         break;
         CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
 #undef CK_CASE
@@ -544,6 +576,30 @@ extern "C" int32_t ck_llr_separable_rows(const void* x, int64_t rows, int64_t ro
                                            row_len)
                   : launch_separable<false>(x, rows * row_len, re, im, 1.0, exact, out, stream,
                                             inv_s2, row_len);
+}
+
+extern "C" int32_t ck_llr_separable_permuted(const void* x, int64_t n, const ck_axis* re,
+                                             const ck_axis* im, double inv_s2, int32_t closed,
+                                             int32_t exact, float* out, const int32_t* perm,
+                                             int64_t period, int64_t limit, void* stream) {
+    CK_REQUIRE(closed == 0 || closed == 1, "closed must be 0 or 1, got %d", closed);
+    CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
+    CK_REQUIRE(!(closed && exact), "the closed form is max-log only");
+    CK_REQUIRE(n >= 0 && re != nullptr, "n = %lld or NULL axis", static_cast<long long>(n));
+    CK_REQUIRE(period >= 1 && limit >= 0 && limit % period == 0,
+               "limit %lld is not a whole number of blocks of %lld", static_cast<long long>(limit),
+               static_cast<long long>(period));
+    CK_REQUIRE(limit <= n * 2 * re->num_bits, "limit %lld past the %lld LLRs",
+               static_cast<long long>(limit), static_cast<long long>(n * 2 * re->num_bits));
+    if (n == 0 || limit == 0) return CK_OK;
+    CK_REQUIRE(perm != nullptr, "NULL perm");
+    // Only the symbols holding LLRs below the limit are demapped.
+    const int64_t k = 2 * re->num_bits;
+    const int64_t m = (limit + k - 1) / k;
+    return closed ? launch_separable<true>(x, m, re, im, inv_s2, 0, out, stream, nullptr, 1, perm,
+                                           period, limit)
+                  : launch_separable<false>(x, m, re, im, inv_s2, exact, out, stream, nullptr, 1,
+                                            perm, period, limit);
 }
 
 extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,

@@ -208,6 +208,53 @@ def llr_any(c: "Constellation", x: Any, noise_var: float, method: str) -> Any:
     return out.reshape(*shape[:-1], shape[-1] * k) if len(shape) else out.reshape(-1)
 
 
+def llr_permuted(c: "Constellation", x: Any, noise_var: float, method: str, perm: Any,
+                 blocks: int, out: Any = None) -> Any:
+    """LLRs of device symbols ``x`` written straight into ``blocks`` blocks of
+    ``P = perm.numel()`` LLRs in a code's bit layout: the LLR of flat index
+    ``t`` (as ``llr_any`` orders them) lands at ``[t // P, perm[t % P]]`` of
+    the ``(blocks, P)`` float32 result; symbols past the blocks are not
+    demapped.  Equals ``llr_any(c, x, noise_var, method)[:blocks * P]
+    .view(blocks, P)[:, iperm]`` with ``iperm`` the inverse of ``perm``,
+    without the gather's extra pass.  Separable constellations; ``perm`` an
+    int32 device permutation (not checked)."""
+    import torch
+
+    check_method(method)
+    if c._axes is None:
+        raise ValueError("llr_permuted: the constellation is not separable per axis")
+    if not x.is_complex():
+        raise ValueError("llr_permuted: complex symbols required")
+    period = int(perm.numel())
+    if perm.dtype != torch.int32 or perm.device != x.device or not perm.is_contiguous():
+        raise ValueError("llr_permuted: perm must be a contiguous int32 tensor on the "
+                         "symbols' device")  # fmt: skip
+    k = c.bits_per_symbol
+    flat = x.reshape(-1).to(torch.complex64).contiguous()
+    limit = int(blocks) * period
+    if limit > flat.numel() * k:
+        raise ValueError(f"llr_permuted: {blocks} blocks of {period} LLRs need "
+                         f"{-(-limit // k)} symbols, got {flat.numel()}")  # fmt: skip
+    if out is None:
+        out = torch.empty((int(blocks), period), dtype=torch.float32, device=x.device)
+    elif (tuple(out.shape) != (int(blocks), period) or out.dtype != torch.float32
+          or not out.is_contiguous() or out.device != x.device):
+        raise ValueError(f"llr_permuted: out must be a contiguous ({blocks}, {period}) "
+                         "float32 tensor")  # fmt: skip
+    if limit == 0:
+        return out
+    inv_s2 = 1.0 / max(noise_var, 1e-20)
+    lib = _native.lib()
+    with torch.cuda.device(x.device):
+        re_ax, im_ax, _keep = axis_tables(c, x.device)
+        closed = closed_form(c._axes, method)
+        _native.check(lib.ck_llr_separable_permuted(
+            flat.data_ptr(), flat.numel(), ctypes.byref(re_ax), ctypes.byref(im_ax),
+            ctypes.c_double(inv_s2), int(closed), int(method == "exact"), out.data_ptr(),
+            perm.data_ptr(), period, limit, _native.stream_of(x.device)))  # fmt: skip
+    return out
+
+
 def llr_rows(c: "Constellation", x: Any, noise_var: Any, method: str, out: Any = None, *,
              inv_s2: Any = None) -> Any:
     """LLRs of ``(R, S)`` complex device symbols, row ``r`` with noise variance ``noise_var[r]``.

@@ -97,6 +97,19 @@ int32_t ck_llr_separable_rows(const void* x, int64_t rows, int64_t row_len,
                               const double* inv_s2, int32_t closed, int32_t exact,
                               float* out, void* stream);
 
+/* Separable LLRs written in a code's bit layout: the LLR of flat index t
+ * (symbol-major, k adjacent) goes to out[(t / period) * period +
+ * perm[t % period]], for t < limit (a whole number of blocks of `period`
+ * LLRs; later symbols are not demapped).  perm (period,) int32 device must
+ * be a permutation of 0..period-1 (not checked).  closed / exact as for
+ * ck_llr_separable_rows.  No counterpart in the reference, which demaps and
+ * then gathers (commkit/coding/shaping.py, PAS): probabilistic amplitude
+ * shaping's codeword LLRs come out of the demapper in one pass. */
+int32_t ck_llr_separable_permuted(const void* x, int64_t n, const ck_axis* re,
+                                  const ck_axis* im, double inv_s2, int32_t closed,
+                                  int32_t exact, float* out, const int32_t* perm,
+                                  int64_t period, int64_t limit, void* stream);
+
 /* LLRs from the metrics of all M points (any constellation).
  * points: (M,) complex128 device; log_prior: (M,) float64 device (zeros =
  * uniform); labels: (M,) int32 packed MSB-first labels; 2 <= M <= 65536,

@@ -210,3 +210,39 @@ def test_llr_rows_inverse_variances_in_a_graph(gpu):
         llr_rows(c, xt, nv, "maxlog")  # eager work between replays
     with pytest.raises(ValueError, match="inv_s2"):
         llr_rows(c, xt, None, "maxlog", inv_s2=inv.float())
+
+
+@pytest.mark.parametrize(("make", "method", "period", "blocks"), [
+    (lambda: Constellation.qam(256).shaped(nu=0.05), "exact", 21600, 3),
+    (lambda: Constellation.qam(16), "maxlog", 1000, 7),
+    (lambda: Constellation.qam(64), "maxlog", 66, 40),
+], ids=["256qam-shaped-exact-pas", "16qam-closed-ragged", "64qam-small-period"])
+def test_llr_permuted_equals_demap_then_gather(gpu, make, method, period, blocks):
+    """The permuted store gives the gathered LLRs bit for bit, for blocks that
+    end inside a symbol and blocks shorter than a tile; symbols past the
+    blocks are left alone."""
+    import torch
+
+    from commkit_amd.mapping.demap import llr_any, llr_permuted
+
+    c = make()
+    k = c.bits_per_symbol
+    rng = np.random.default_rng(period)
+    nsym = -(-blocks * period // k) + 5
+    x = _noisy(c, nsym, 0.02, seed=period)
+    xt = torch.from_numpy(x).cuda()
+    if period == 21600:
+        from commkit_amd.coding import PAS
+        from commkit_amd.workloads import seeded_code
+
+        perm = PAS(seeded_code("r5/6-z360"), c)._tables(xt.device)[0]
+    else:
+        perm = torch.from_numpy(rng.permutation(period)).cuda()
+    iperm = torch.argsort(perm)
+    want = llr_any(c, xt, 0.02, method)[: blocks * period].view(blocks, period)[:, iperm]
+    got = llr_permuted(c, xt, 0.02, method, perm.to(torch.int32).contiguous(), blocks)
+    assert torch.equal(got, want)
+    with pytest.raises(ValueError, match="int32"):
+        llr_permuted(c, xt, 0.02, method, perm, blocks)
+    with pytest.raises(ValueError, match="symbols"):
+        llr_permuted(c, xt, 0.02, method, perm.to(torch.int32), blocks + 100)
