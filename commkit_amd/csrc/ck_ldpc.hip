@@ -11,9 +11,9 @@
 // mag = max(scale * min - offset, 0) (min2 for edge idx) or
 // phi(total - phi(|Q|)); the quasi-cyclic kernels evaluate phi from the
 // hardware exp2 / log2 (phi_fast, within the sum-product tolerance), the
-// generic kernel with logf / tanhf (with phi_fast, this compiler rejects
-// the generic kernels' flat LDS/HBM addressing).  This translation unit is
-// compiled with
+// generic kernel with logf / tanhf (with phi_fast, this hipcc stops on an
+// illegal-operand error while selecting the generic kernels' instructions).
+// This translation unit is compiled with
 // -ffp-contract=off, so every product and sum rounds on its own, as in the
 // reference: min-sum is bit-identical to the CPU restatement, including
 // iteration counts and valid flags.  Early stop checks every row's parity of
@@ -236,18 +236,28 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
 // ---------------------------------------------------------------------------
 
 // The APP vector of the generic kernel: variables below `split` in LDS, the
-// rest in the frame's HBM workspace `glob` (ALL_LDS: all of it in LDS).  A
-// flat address serves lanes of either kind in one load instruction, so a
-// frame too long for LDS keeps its lowest-numbered variables there (for the
-// systematic codes here, the information bits: the columns of highest degree).
+// rest in the frame's HBM workspace `glob` (ALL_LDS: all of it in LDS); a
+// lane's access goes to one or the other (an LDS and a global instruction,
+// each under its lanes' mask), so a frame too long for LDS keeps its
+// lowest-numbered variables there (for the systematic codes here, the
+// information bits: the columns of highest degree).
 template <bool ALL_LDS>
 struct GenApp {
     float* lds;
     float* glob;
     int split;
-    __device__ __forceinline__ float* at(int v) const {
-        if constexpr (ALL_LDS) return lds + v;
-        return v < split ? lds + v : glob + v;
+    __device__ __forceinline__ float ld(int v) const {
+        if constexpr (ALL_LDS) return lds[v];
+        return v < split ? lds[v] : glob[v];
+    }
+    __device__ __forceinline__ void st(int v, float x) const {
+        if constexpr (ALL_LDS) {
+            lds[v] = x;
+        } else if (v < split) {
+            lds[v] = x;
+        } else {
+            glob[v] = x;
+        }
     }
 };
 
@@ -275,11 +285,11 @@ __device__ bool generic_syndrome_ok(const ck_ldpc_code& c, const App& A) {
 #pragma unroll
                 for (int i = 0; i < D; ++i) col[i] = c.cols[e0 + i < e1 ? e0 + i : last];
 #pragma unroll
-                for (int i = 0; i < D; ++i) v[i] = *A.at(col[i]);
+                for (int i = 0; i < D; ++i) v[i] = A.ld(col[i]);
 #pragma unroll
                 for (int i = 0; i < D; ++i) parity ^= (e0 + i < e1 && v[i] < 0.0f) ? 1 : 0;
             } else {
-                for (int e = e0; e < e1; ++e) parity ^= *A.at(c.cols[e]) < 0.0f;
+                for (int e = e0; e < e1; ++e) parity ^= A.ld(c.cols[e]) < 0.0f;
             }
         }
         if (__syncthreads_or(parity)) return false;
@@ -340,7 +350,7 @@ __device__ __forceinline__ void row_load_q(const App& A, const float* __restrict
 #pragma unroll
     for (int i = 0; i < MAXD; ++i) {
         const bool in = r.col[i] >= 0;
-        av[i] = *A.at(in ? r.col[i] : 0);
+        av[i] = A.ld(in ? r.col[i] : 0);
         rv[i] = first ? 0.0f : Rm[in ? r.base + i * r.stride : r.base];
     }
 #pragma unroll
@@ -364,7 +374,7 @@ __device__ __forceinline__ void row_update(const App& A, float* __restrict__ Rm,
             if (r.col[i] >= 0) {
                 const float mag = phi(total - phi(fabsf(r.q[i])));
                 const float rn = (neg ^ ((r.q[i] < 0.0f) ? 1u : 0u)) ? -mag : mag;
-                if (UPDATE_APP) *A.at(r.col[i]) = r.q[i] + rn;
+                if (UPDATE_APP) A.st(r.col[i], r.q[i] + rn);
                 Rm[r.base + i * r.stride] = rn;
             }
         return;
@@ -392,7 +402,7 @@ __device__ __forceinline__ void row_update(const App& A, float* __restrict__ Rm,
         if (r.col[i] >= 0) {
             const float mag = (i == idx) ? m2 : m1;
             const float rn = (neg ^ ((r.q[i] < 0.0f) ? 1u : 0u)) ? -mag : mag;
-            if (UPDATE_APP) *A.at(r.col[i]) = r.q[i] + rn;
+            if (UPDATE_APP) A.st(r.col[i], r.q[i] + rn);
             Rm[r.base + i * r.stride] = rn;
         }
 }
@@ -438,16 +448,16 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, const App& A,
     if (SP) {
         float total = 0.0f;
         for (int e = e0; e < e1; ++e) {
-            const float q = *A.at(c.cols[e]) - (first ? 0.0f : Rm[e]);
+            const float q = A.ld(c.cols[e]) - (first ? 0.0f : Rm[e]);
             neg ^= (q < 0.0f) ? 1u : 0u;
             total = total + phi(fabsf(q));
         }
         for (int e = e0; e < e1; ++e) {
             const int col = c.cols[e];
-            const float q = *A.at(col) - (first ? 0.0f : Rm[e]);
+            const float q = A.ld(col) - (first ? 0.0f : Rm[e]);
             const float mag = phi(total - phi(fabsf(q)));
             const float rn = (neg ^ ((q < 0.0f) ? 1u : 0u)) ? -mag : mag;
-            if (UPDATE_APP) *A.at(col) = q + rn;
+            if (UPDATE_APP) A.st(col, q + rn);
             Rm[e] = rn;
         }
         return;
@@ -456,7 +466,7 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, const App& A,
     float min2 = ck::kInf;
     int idx = -1;
     for (int e = e0; e < e1; ++e) {
-        const float q = *A.at(c.cols[e]) - (first ? 0.0f : Rm[e]);
+        const float q = A.ld(c.cols[e]) - (first ? 0.0f : Rm[e]);
         neg ^= (q < 0.0f) ? 1u : 0u;
         const float mag = fabsf(q);
         if (mag < min1) {
@@ -471,10 +481,10 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, const App& A,
     const float m2 = fmaxf(prm.scale * min2 - prm.offset, 0.0f);
     for (int e = e0; e < e1; ++e) {
         const int col = c.cols[e];
-        const float q = *A.at(col) - (first ? 0.0f : Rm[e]);
+        const float q = A.ld(col) - (first ? 0.0f : Rm[e]);
         const float mag = (e == idx) ? m2 : m1;
         const float rn = (neg ^ ((q < 0.0f) ? 1u : 0u)) ? -mag : mag;
-        if (UPDATE_APP) *A.at(col) = q + rn;
+        if (UPDATE_APP) A.st(col, q + rn);
         Rm[e] = rn;
     }
 }
@@ -492,7 +502,7 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
     const GenApp<APP_LDS> A{lds, glob, APP_LDS ? N : lds_vars};
     float* Rm = msg_ws + static_cast<size_t>(frame) * msg_stride;
     const float* L = llr + frame * c.n;
-    for (int v = threadIdx.x; v < N; v += blockDim.x) *A.at(v) = v < c.n ? L[v] : 0.0f;
+    for (int v = threadIdx.x; v < N; v += blockDim.x) A.st(v, v < c.n ? L[v] : 0.0f);
     __syncthreads();
 
     int it = 0;
@@ -506,7 +516,7 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
             for (int v = threadIdx.x; v < N; v += blockDim.x) {
                 float sum = v < c.n ? L[v] : 0.0f;
                 for (int t = c.var_ptr[v]; t < c.var_ptr[v + 1]; ++t) sum = sum + Rm[c.var_edges[t]];
-                *A.at(v) = sum;
+                A.st(v, sum);
             }
             __syncthreads();
         } else {
