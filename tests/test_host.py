@@ -9,7 +9,7 @@ from commkit_amd.coding.gf2 import inverse, rank, row_reduce
 from commkit_amd.workloads import CODES, seeded_base, seeded_code
 
 from .codes import SMALL_QC_BASE, TREE_ROWS, encode_host, h_of, rows_in_schedule, small_qc, syndrome
-from .codes import quiet, tree_code
+from .codes import tree_code
 
 
 class TestCodes:
