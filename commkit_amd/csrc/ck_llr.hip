@@ -100,7 +100,18 @@ struct AxisTab {
     short lo[KB][L];
     short hi[KB][L];
     int pos[KB];
+    // Level-metric form only: nonzero when the labels are the reflected Gray
+    // code up to a constant XOR (gray_xor), the levels ascend equally spaced
+    // and the log prior is concave over them.  Then the metric
+    // -(x - l)^2 inv_s2 + logp(l) is concave in the level index for every x
+    // and inv_s2 > 0, and each half's best level is the peak or one of the
+    // peak's two nearest levels of opposite bit (lo/hi, built in the kernel).
+    int unimodal_gray;
+    int gray_xor;
 };
+
+// Reflected Gray label of level l.
+__host__ __device__ constexpr int gray_of(int l) { return l ^ (l >> 1); }
 
 template <int KB>
 __device__ void stage_axis(AxisTab<KB>& t, const ck_axis& a, bool with_opposite) {
@@ -119,6 +130,43 @@ __device__ void stage_axis(AxisTab<KB>& t, const ck_axis& a, bool with_opposite)
         }
     }
     if (threadIdx.x < KB) t.pos[threadIdx.x] = a.pos[threadIdx.x];
+    if (with_opposite) return;
+    __syncthreads();
+    // Opposite levels from the staged labels (same definition as the closed
+    // form's tables): nearest level below / above s whose bit b differs.
+    for (int i = threadIdx.x; i < KB * L; i += blockDim.x) {
+        const int b = i / L;
+        const int s = i % L;
+        const int sh = KB - 1 - b;
+        const int bit = (t.lab[s] >> sh) & 1;
+        int lo = -1;
+        for (int j = s - 1; j >= 0; --j)
+            if (((t.lab[j] >> sh) & 1) != bit) {
+                lo = j;
+                break;
+            }
+        int hi = -1;
+        for (int j = s + 1; j < L; ++j)
+            if (((t.lab[j] >> sh) & 1) != bit) {
+                hi = j;
+                break;
+            }
+        t.lo[b][s] = static_cast<short>(lo);
+        t.hi[b][s] = static_cast<short>(hi);
+    }
+    if (threadIdx.x == 0) {
+        const int x0 = t.lab[0];
+        const double step = t.lev[1] - t.lev[0];
+        bool ok = step > 0.0;
+        for (int l = 0; l < L && ok; ++l) {
+            ok = ok && ((t.lab[l] ^ gray_of(l)) == x0);
+            if (l + 1 < L) ok = ok && fabs((t.lev[l + 1] - t.lev[l]) - step) <= 1e-9 * step;
+            if (l >= 1 && l + 1 < L)
+                ok = ok && (t.logp[l - 1] + t.logp[l + 1] - 2.0 * t.logp[l]) <= 1e-9;
+        }
+        t.unimodal_gray = ok ? 1 : 0;
+        t.gray_xor = x0;
+    }
 }
 
 // m_a - m_c of two levels of one axis, in float64, factored (see the file
@@ -179,6 +227,141 @@ __device__ __forceinline__ void axis_closed(const AxisTab<KB>& t, float x, doubl
 constexpr double kWeak = -80.0;
 constexpr float kLog2e = 1.4426950408889634f;
 
+// m_a - m_c in the factored form of level_diff, with the float64 level and
+// log prior of c and x - l_c passed in (taken once per symbol).
+__device__ __forceinline__ double diff_to(double la, double pa, double xd, double lc, double pc,
+                                          double xc, double inv_s2) {
+    return (la - lc) * ((xd - la) + xc) * inv_s2 + (pa - pc);
+}
+
+// Below this (in nats) under the peak, the other half's best level is summed
+// relative to itself: hardware exp2 of a term more than 126 octaves under the
+// peak returns zero, which against a best term above exp(-60) leaves out at
+// most exp(-27) of the half.
+constexpr double kWeakGray = -60.0;
+
+// log(1 + a) - log(1 + b) for a, b >= 0 as one logarithm: oriented so that
+// q = (max - min) / (1 + min) >= 0, and log(1 + q) = log(u) + r / u with
+// u + r = 1 + q exactly (TwoSum), so the result carries about one rounding
+// of logf instead of two of log1pf (ocml's log1pf is a compensated
+// double-float routine, several times the cost of logf).
+__device__ __forceinline__ float log1p_diff(float a, float b) {
+    const bool neg = a < b;
+    const float hi = neg ? b : a;
+    const float lo = neg ? a : b;
+    const float q = (hi - lo) / (1.0f + lo);
+    const float u = 1.0f + q;
+    const float bv = u - 1.0f;
+    const float r = (1.0f - (u - bv)) + (q - bv);
+    const float d = logf(u) + r * __builtin_amdgcn_rcpf(u);
+    return neg ? -d : d;
+}
+
+// LLRs of one axis whose tables are flagged unimodal_gray (levels ascending
+// and equally spaced, log prior concave, labels Gray up to a constant XOR).
+// The float32 metrics locate the peak level pk; for each bit the half that
+// holds pk has pk as its best level, and the other half's best level a_o is
+// the better, in float64 metric, of pk's two nearest levels of opposite bit.
+// Max-log: LLR = -/+ (m_o - m_pk).  Exact: every other level's term is
+// exp(m_l - m_pk); T_own sums pk's half without pk, U_oth the other half
+// without a_o, and LLR = (m_pk - m_o) + log(1 + T_own) - log(1 + U_oth / e_o)
+// with e_o = exp(m_o - m_pk), sign by the half of pk: the float64 difference
+// and log(1 + T) terms of the generic form, the two logarithms taken as one
+// by log1p_diff.  Half membership is compile-time: Gray bit b of level l is
+// a constant of the unrolled loops.
+template <int KB>
+__device__ __forceinline__ void axis_levels_gray(const AxisTab<KB>& t, float x, float inv_s2f,
+                                                 double inv_s2, bool exact, float* row) {
+    constexpr int L = 1 << KB;
+    float best = -ck::kInf;
+    int pk = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const float dx = x - t.levf[l];
+        const float m = -(dx * dx) * inv_s2f + t.logpf[l];
+        if (m > best) {
+            best = m;
+            pk = l;
+        }
+    }
+    const double xd = x;
+    const double lpk = t.lev[pk];
+    const double ppk = t.logp[pk];
+    const double xpk = xd - lpk;
+    const int gpk = gray_of(pk);
+    int ao[KB];
+    double wo[KB];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        const int jl = t.lo[b][pk];
+        const int jh = t.hi[b][pk];
+        double wl = -1.0e300;
+        double wh = -1.0e300;
+        if (jl >= 0) wl = diff_to(t.lev[jl], t.logp[jl], xd, lpk, ppk, xpk, inv_s2);
+        if (jh >= 0) wh = diff_to(t.lev[jh], t.logp[jh], xd, lpk, ppk, xpk, inv_s2);
+        ao[b] = (wh > wl) ? jh : jl;
+        wo[b] = (wh > wl) ? wh : wl;
+    }
+    if (!exact) {
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            const int sh = KB - 1 - b;
+            const float r = static_cast<float>(-wo[b]);
+            row[t.pos[b]] = ((((gpk ^ t.gray_xor) >> sh) & 1) != 0) ? -r : r;
+        }
+        return;
+    }
+    float s0[KB], s1[KB];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        s0[b] = 0.0f;
+        s1[b] = 0.0f;
+    }
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const double w = diff_to(t.lev[l], t.logp[l], xd, lpk, ppk, xpk, inv_s2);
+        float e = __builtin_amdgcn_exp2f(static_cast<float>(w) * kLog2e);
+        e = (l == pk) ? 0.0f : e;
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            const float eb = (l == ao[b]) ? 0.0f : e;
+            if ((gray_of(l) >> (KB - 1 - b)) & 1)
+                s1[b] += eb;
+            else
+                s0[b] += eb;
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        const int sh = KB - 1 - b;
+        const int own = (gpk >> sh) & 1;
+        const float t_own = own ? s1[b] : s0[b];
+        float t_oth;
+        if (wo[b] >= kWeakGray) {
+            t_oth = (own ? s0[b] : s1[b]) /
+                    __builtin_amdgcn_exp2f(static_cast<float>(wo[b]) * kLog2e);
+        } else {
+            // The other half lies far under the peak: its terms relative to
+            // its own best level a_o.
+            const int a = ao[b];
+            const double la = t.lev[a];
+            const double pa = t.logp[a];
+            const double xa = xd - la;
+            t_oth = 0.0f;
+#pragma unroll 1
+            for (int l = 0; l < L; ++l) {
+                if (((gray_of(l) >> sh) & 1) != own && l != a)
+                    t_oth += exp2f(
+                        static_cast<float>(diff_to(t.lev[l], t.logp[l], xd, la, pa, xa, inv_s2)) *
+                        kLog2e);
+            }
+        }
+        const double v = -wo[b] + static_cast<double>(log1p_diff(t_own, t_oth));
+        const float r = static_cast<float>(v);
+        row[t.pos[b]] = ((own ^ (t.gray_xor >> sh)) & 1) ? -r : r;
+    }
+}
+
 // LLRs of one axis from the metrics of its levels into row[pos[b]].
 // Max-log: the difference of the two halves' maxima, located in float64.
 // Exact: LLR = (m_a0 - m_a1) + log(1 + T0) - log(1 + T1), with a_h the
@@ -190,6 +373,12 @@ constexpr float kLog2e = 1.4426950408889634f;
 template <int KB>
 __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float inv_s2f,
                                             double inv_s2, bool exact, float* row) {
+    if constexpr (KB <= 5) {
+        if (t.unimodal_gray) {
+            axis_levels_gray<KB>(t, x, inv_s2f, inv_s2, exact, row);
+            return;
+        }
+    }
     constexpr int L = 1 << KB;
     const double xd = x;
     int a0[KB], a1[KB];

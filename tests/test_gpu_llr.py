@@ -246,3 +246,58 @@ def test_llr_permuted_equals_demap_then_gather(gpu, make, method, period, blocks
         llr_permuted(c, xt, 0.02, method, perm, blocks)
     with pytest.raises(ValueError, match="symbols"):
         llr_permuted(c, xt, 0.02, method, perm.to(torch.int32), blocks + 100)
+
+
+def _natural_and_bimodal():
+    """16-QAM variants the unimodal Gray form must not take: natural-binary
+    axis labels (not Gray), and a separable prior whose per-axis log is
+    convex (0.4, 0.1, 0.1, 0.4 over the levels), so the metric along an axis
+    can have two peaks."""
+    from dataclasses import replace
+
+    c = Constellation.qam(16)
+    lev = np.unique(np.round(c.points.real, 12))
+    ii = np.searchsorted(lev, np.round(c.points.real, 12))
+    qq = np.searchsorted(lev, np.round(c.points.imag, 12))
+    lab = np.stack([(ii >> 1) & 1, ii & 1, (qq >> 1) & 1, qq & 1], 1).astype(np.int8)
+    p1 = np.array([0.4, 0.1, 0.1, 0.4])
+    pmf = p1[ii] * p1[qq]
+    return replace(c, bit_labels=lab), replace(c, pmf=pmf / pmf.sum())
+
+
+GRAY_SWEEP = {
+    "16qam": (lambda: Constellation.qam(16), [5.0, 10.0, 20.0, 30.0, 40.0]),
+    "64qam": (lambda: Constellation.qam(64), [10.0, 15.0, 20.0, 30.0, 40.0]),
+    "256qam": (lambda: Constellation.qam(256), [10.0, 15.0, 20.0, 30.0, 40.0]),
+    "256qam-shaped": (lambda: Constellation.qam(256).shaped(nu=0.05), [10.0, 13.7, 20.0, 30.0]),
+    "1024qam": (lambda: Constellation.qam(1024), [25.0, 30.0, 40.0]),
+    "16qam-natural": (lambda: _natural_and_bimodal()[0], [5.0, 15.0, 30.0]),
+    "16qam-bimodal-prior": (lambda: _natural_and_bimodal()[1], [5.0, 15.0, 30.0]),
+}
+
+
+@pytest.mark.parametrize("name", list(GRAY_SWEEP))
+@pytest.mark.parametrize("method", ["maxlog", "exact"])
+def test_llr_sweep_against_fp64(gpu, name, method):
+    """Separable level metrics over an Es/N0 sweep: Gray axes with a concave
+    (uniform or Maxwell-Boltzmann) log prior take the unimodal Gray form,
+    whose far-under halves (high Es/N0) are summed relative to their own best
+    level; the natural-label and two-peak-prior cases take the general form.
+    Same tolerance as above on every element."""
+    import torch
+
+    make, dbs = GRAY_SWEEP[name]
+    c = make()
+    if method == "maxlog" and c.pmf is None and name != "16qam-natural":
+        pytest.skip("uniform Gray max-log takes the closed form (tested above)")
+    for db in dbs:
+        nv = 10 ** (-db / 10)
+        x = _noisy(c, 4096 if c.order >= 1024 else 8192, nv, seed=int(db * 10) + 7)
+        got = compute_llr(torch.from_numpy(x).cuda(), noise_var=nv, constellation=c,
+                          method=method).cpu().numpy()  # fmt: skip
+        ref = llr_fp64(x, c.points, c.bit_labels, c.pmf, nv, method)
+        ok, under = llr_tolerance_ok(got, ref, floor=FLOOR[method])
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), max(FLOOR[method], 1e-30))
+        print(f"{name} {method} {db} dB: worst error / bar {rel.max() / 1e-5:.3f}")
+        assert ok, f"{db} dB: worst error {rel.max() / 1e-5:.3f} x the bar"
+        assert np.array_equal(np.sign(got[~under]), np.sign(ref[~under]))
