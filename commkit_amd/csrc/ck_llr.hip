@@ -65,25 +65,39 @@ __device__ __forceinline__ void store_tile(const float* tile, float* __restrict_
 // goes to out[(t / period) * period + perm[t % period]] and indices from
 // `limit` on are dropped: the bit layout of a code's blocks (probabilistic
 // amplitude shaping) applied on the way out.  A tile may span several blocks.
+// Lane i handles tile elements i, i + kThreads, ... (K of them at most): it
+// gathers all their permutation entries first, so the K loads are in flight
+// together, then stores.
+template <int K>
 __device__ __forceinline__ void store_tile_perm(const float* tile, float* __restrict__ out,
-                                                long long base, int count, int k,
+                                                long long base, int count,
                                                 const int* __restrict__ perm, long long period,
                                                 long long limit) {
-    const int stride = tile_stride(k);
-    const long long t0 = base * k;
+    const int stride = tile_stride(K);
+    const long long t0 = base * K;
+    if (t0 >= limit) return;
     const long long blk0 = t0 / period;
-    const long long off0 = t0 - blk0 * period;
-    const int total = count * k;
-    for (int j = threadIdx.x; j < total; j += kThreads) {
-        if (t0 + j >= limit) break;  // j grows: every later index is past it too
-        long long off = off0 + j;
-        long long blk = blk0;
-        while (off >= period) {
+    int total = count * K;
+    if (limit - t0 < total) total = static_cast<int>(limit - t0);
+    long long off = t0 - blk0 * period + threadIdx.x;
+    long long blk = blk0;
+    long long dst[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        while (off >= period) {  // period may be shorter than a tile
             off -= period;
             ++blk;
         }
-        const int sym = j / k;
-        out[blk * period + perm[off]] = tile[sym * stride + (j - sym * k)];
+        if (threadIdx.x + u * kThreads < total) dst[u] = blk * period + perm[off];
+        off += kThreads;
+    }
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        const int j = threadIdx.x + u * kThreads;
+        if (j < total) {
+            const int sym = j / K;
+            out[dst[u]] = tile[sym * stride + (j - sym * K)];
+        }
     }
 }
 
@@ -520,7 +534,7 @@ __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
         const long long left = n - base;
         const int count = left < kThreads ? static_cast<int>(left) : kThreads;
         if (perm != nullptr)
-            store_tile_perm(tile, out, base, count, K, perm, period, limit);
+            store_tile_perm<K>(tile, out, base, count, perm, period, limit);
         else
             store_tile(tile, out, base, count, K);
         __syncthreads();
