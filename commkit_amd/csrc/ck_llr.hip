@@ -489,7 +489,7 @@ __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float
             t1 = u1[b] / exp2f(static_cast<float>(w1[b]) * kLog2e);
         }
         const double d01 = level_diff(t, xd, inv_s2, a0[b], a1[b]);
-        row[t.pos[b]] = static_cast<float>(d01 + static_cast<double>(log1pf(t0) - log1pf(t1)));
+        row[t.pos[b]] = static_cast<float>(d01 + static_cast<double>(log1p_diff(t0, t1)));
     }
 }
 
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(kThreads) void llr_points_kernel(
                         t1 = u1[b] / exp2f(static_cast<float>(w1[b]) * kLog2e);
                     }
                     const double d01 = point_diff(pts, logp, xr, xi, inv_s2, a0[b], a1[b]);
-                    row[b] = static_cast<float>(d01 + static_cast<double>(log1pf(t0) - log1pf(t1)));
+                    row[b] = static_cast<float>(d01 + static_cast<double>(log1p_diff(t0, t1)));
                 }
             }
         }
