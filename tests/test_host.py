@@ -251,3 +251,53 @@ class TestArgumentChecks:
             decode(np.zeros(40, np.float32), code="not a code")
         with pytest.raises(TypeError, match="does not decode"):
             decode(np.zeros(40, np.float32), code=code, method=object())
+
+
+def _rank_mod2(m: np.ndarray) -> int:
+    """Rank over GF(2) by plain Gaussian elimination on Python ints (one int
+    per row, an independent check of gf2.rank)."""
+    rows = [int("".join(str(int(b)) for b in r), 2) for r in np.asarray(m) & 1] if m.size else []
+    rank = 0
+    while rows:
+        pivot = max(rows)
+        rows.remove(pivot)
+        if pivot == 0:
+            break
+        top = pivot.bit_length() - 1
+        rows = [r ^ pivot if (r >> top) & 1 else r for r in rows]
+        rank += 1
+    return rank
+
+
+class TestPivotChoice:
+    """``row_reduce`` pivots, which fix the dense encoder's information
+    positions (``commkit/coding/_gf2.py:45-88``, ``ldpc.py:445-481``): a
+    column is a pivot exactly when it is independent of the columns taken
+    before it, in ascending order (forward) or from the last column back
+    (reverse) -- the greedy basis of the column matroid, checked here against
+    ranks of column ranges."""
+
+    @pytest.mark.parametrize("seed", range(6))
+    def test_pivots_are_the_greedy_basis(self, seed):
+        rng = np.random.default_rng(seed)
+        r, n = [(4, 9), (6, 6), (7, 12), (5, 16), (9, 14), (3, 20)][seed]
+        m = (rng.random((r, n)) < 0.35).astype(np.int8)
+        if seed == 1:
+            m[2] = m[0] ^ m[1]  # a dependent row
+        _, fwd = row_reduce(m)
+        _, rev = row_reduce(m, reverse=True)
+        want_fwd = [c for c in range(n) if _rank_mod2(m[:, : c + 1]) > _rank_mod2(m[:, :c])]
+        want_rev = [c for c in range(n - 1, -1, -1)
+                    if _rank_mod2(m[:, c:]) > _rank_mod2(m[:, c + 1 :])]  # fmt: skip
+        assert fwd.tolist() == want_fwd
+        assert rev.tolist() == want_rev
+        assert len(want_fwd) == len(want_rev) == _rank_mod2(m) == rank(m)
+
+    def test_dense_encoder_information_positions(self):
+        code = tree_code()
+        assert code._encoder.kind == "dense"
+        h = code.parity_check().toarray().astype(np.int8)
+        n = h.shape[1]
+        piv = [c for c in range(n - 1, -1, -1) if _rank_mod2(h[:, c:]) > _rank_mod2(h[:, c + 1 :])]
+        np.testing.assert_array_equal(code._encoder.info, np.setdiff1d(np.arange(n), piv))
+
