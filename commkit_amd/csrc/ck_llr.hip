@@ -119,7 +119,7 @@ struct AxisTab {
     // and the log prior is concave over them.  Then the metric
     // -(x - l)^2 inv_s2 + logp(l) is concave in the level index for every x
     // and inv_s2 > 0, and each half's best level is the peak or one of the
-    // peak's two nearest levels of opposite bit (lo/hi, built in the kernel).
+    // peak's two nearest levels of opposite bit (gray_opposite).
     int unimodal_gray;
     int gray_xor;
 };
@@ -146,41 +146,37 @@ __device__ void stage_axis(AxisTab<KB>& t, const ck_axis& a, bool with_opposite)
     if (threadIdx.x < KB) t.pos[threadIdx.x] = a.pos[threadIdx.x];
     if (with_opposite) return;
     __syncthreads();
-    // Opposite levels from the staged labels (same definition as the closed
-    // form's tables): nearest level below / above s whose bit b differs.
-    for (int i = threadIdx.x; i < KB * L; i += blockDim.x) {
-        const int b = i / L;
-        const int s = i % L;
-        const int sh = KB - 1 - b;
-        const int bit = (t.lab[s] >> sh) & 1;
-        int lo = -1;
-        for (int j = s - 1; j >= 0; --j)
-            if (((t.lab[j] >> sh) & 1) != bit) {
-                lo = j;
-                break;
-            }
-        int hi = -1;
-        for (int j = s + 1; j < L; ++j)
-            if (((t.lab[j] >> sh) & 1) != bit) {
-                hi = j;
-                break;
-            }
-        t.lo[b][s] = static_cast<short>(lo);
-        t.hi[b][s] = static_cast<short>(hi);
-    }
-    if (threadIdx.x == 0) {
-        const int x0 = t.lab[0];
+    // One thread per level (L <= blockDim): Gray label up to lab[0], equal
+    // spacing, concave log prior around it.
+    bool ok = true;
+    const int l = threadIdx.x;
+    if (l < L) {
         const double step = t.lev[1] - t.lev[0];
-        bool ok = step > 0.0;
-        for (int l = 0; l < L && ok; ++l) {
-            ok = ok && ((t.lab[l] ^ gray_of(l)) == x0);
-            if (l + 1 < L) ok = ok && fabs((t.lev[l + 1] - t.lev[l]) - step) <= 1e-9 * step;
-            if (l >= 1 && l + 1 < L)
-                ok = ok && (t.logp[l - 1] + t.logp[l + 1] - 2.0 * t.logp[l]) <= 1e-9;
-        }
-        t.unimodal_gray = ok ? 1 : 0;
-        t.gray_xor = x0;
+        ok = step > 0.0 && (t.lab[l] ^ gray_of(l)) == t.lab[0];
+        if (l + 1 < L) ok = ok && fabs((t.lev[l + 1] - t.lev[l]) - step) <= 1e-9 * step;
+        if (l >= 1 && l + 1 < L)
+            ok = ok && (t.logp[l - 1] + t.logp[l + 1] - 2.0 * t.logp[l]) <= 1e-9;
     }
+    const int all = __syncthreads_and(ok ? 1 : 0);
+    if (threadIdx.x == 0) {
+        t.unimodal_gray = all;
+        t.gray_xor = t.lab[0];
+    }
+}
+
+// Nearest levels below (*lo) and above (*hi) level s whose reflected Gray
+// bit sh differs from s's, -1 for none.  That bit changes value entering
+// the levels l = 2^sh (mod 2^(sh+1)), so s's run of equal bits starts at
+// q 2^(sh+1) - 2^sh (or 0) and the next run at (q + 1) 2^(sh+1) - 2^sh,
+// with q = (s + 2^sh) >> (sh + 1).  A constant XOR on the labels keeps the
+// runs.
+template <int L>
+__device__ __forceinline__ void gray_opposite(int s, int sh, int* lo, int* hi) {
+    const int h = 1 << sh;
+    const int q = (s + h) >> (sh + 1);
+    *lo = q == 0 ? -1 : (q << (sh + 1)) - h - 1;
+    const int nx = ((q + 1) << (sh + 1)) - h;
+    *hi = nx < L ? nx : -1;
 }
 
 // m_a - m_c of two levels of one axis, in float64, factored (see the file
@@ -307,8 +303,8 @@ __device__ __forceinline__ void axis_levels_gray(const AxisTab<KB>& t, float x, 
     double wo[KB];
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
-        const int jl = t.lo[b][pk];
-        const int jh = t.hi[b][pk];
+        int jl, jh;
+        gray_opposite<L>(pk, KB - 1 - b, &jl, &jh);
         double wl = -1.0e300;
         double wh = -1.0e300;
         if (jl >= 0) wl = diff_to(t.lev[jl], t.logp[jl], xd, lpk, ppk, xpk, inv_s2);
@@ -495,11 +491,15 @@ __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float
 
 // CLOSED: closed form (max-log, uniform); else level metrics.
 // inv_rows (optional): 1/sigma^2 per row of row_len symbols, instead of inv_s2.
+#define CK_SEPARABLE_PARAMS                                                                      \
+    const float2 *__restrict__ x, long long n, ck_axis re, ck_axis im, float inv_s2f,           \
+        double inv_s2, int exact, float *__restrict__ out, const double *__restrict__ inv_rows, \
+        long long row_len, const int *__restrict__ perm, long long period, long long limit
+#define CK_SEPARABLE_ARGS \
+    x, n, re, im, inv_s2f, inv_s2, exact, out, inv_rows, row_len, perm, period, limit
+
 template <int KB, bool CLOSED>
-__global__ __launch_bounds__(kThreads) void llr_separable_kernel(
-    const float2* __restrict__ x, long long n, ck_axis re, ck_axis im, float inv_s2f,
-    double inv_s2, int exact, float* __restrict__ out, const double* __restrict__ inv_rows,
-    long long row_len, const int* __restrict__ perm, long long period, long long limit) {
+__device__ __forceinline__ void separable_body(CK_SEPARABLE_PARAMS) {
     constexpr int K = 2 * KB;
     __shared__ AxisTab<KB> tab[2];
     __shared__ float tile[kThreads * (K | 1)];
@@ -539,6 +539,21 @@ __global__ __launch_bounds__(kThreads) void llr_separable_kernel(
             store_tile(tile, out, base, count, K);
         __syncthreads();
     }
+}
+
+template <int KB, bool CLOSED>
+__global__ __launch_bounds__(kThreads) void llr_separable_kernel(CK_SEPARABLE_PARAMS) {
+    separable_body<KB, CLOSED>(CK_SEPARABLE_ARGS);
+}
+
+// Level metrics of axes of up to 16 levels: held to 128 VGPRs, four waves
+// per SIMD, which the form of 256-QAM fits without spilling (unconstrained
+// it takes 136 and three waves; wider axes would spill under the cap, and
+// the closed form needs under 60).
+template <int KB, bool CLOSED>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void
+llr_separable_kernel_w4(CK_SEPARABLE_PARAMS) {
+    separable_body<KB, CLOSED>(CK_SEPARABLE_ARGS);
 }
 
 // m_a - m_c of two points in float64, factored per component.
@@ -715,6 +730,16 @@ int32_t check_axis(const ck_axis* a, const char* name, bool closed) {
     return CK_OK;
 }
 
+// The kernel instantiation for KB bits per axis (only the chosen wrapper is
+// instantiated).
+template <int KB, bool CLOSED>
+constexpr auto separable_kernel() {
+    if constexpr (!CLOSED && KB <= 4)
+        return llr_separable_kernel_w4<KB, CLOSED>;
+    else
+        return llr_separable_kernel<KB, CLOSED>;
+}
+
 template <bool CLOSED>
 int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_axis* im,
                          double inv_s2, int exact, float* out, void* stream,
@@ -735,7 +760,7 @@ int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_a
     switch (re->num_bits) {
 #define CK_CASE(KB)                                                                           \
     case KB:                                                                                  \
-        hipLaunchKernelGGL((llr_separable_kernel<KB, CLOSED>), dim3(grid), dim3(kThreads), 0, \
+        hipLaunchKernelGGL((separable_kernel<KB, CLOSED>()), dim3(grid), dim3(kThreads), 0,   \
                            s, xs, static_cast<long long>(n), *re, *im,                        \
                            static_cast<float>(inv_s2), inv_s2, exact, out, inv_rows,          \
                            static_cast<long long>(row_len), perm,                             \
