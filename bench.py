@@ -61,7 +61,7 @@ def _parse() -> argparse.Namespace:
                     help="serving: launches in flight (one workspace and stream each)")
     ap.add_argument("--bound-load", type=float, default=0.6,
                     help="serving: offered load (fraction of saturated) the p99 bound is stated at")
-    ap.add_argument("--p99-bound-ms", type=float, default=10.0,
+    ap.add_argument("--p99-bound-ms", type=float, default=25.0,
                     help="serving: stated p99 frame-latency bound at --bound-load")
     ap.add_argument("--workload", choices=("single", "mixed", "shaped", "serving"),
                     default="single",
