@@ -4,7 +4,10 @@
 //                     the snapped level against the nearer of the two levels
 //                     just outside its run of equal bits (1 + 2 kb metrics);
 //   separable_levels  the metrics of each axis's levels with the log prior,
-//                     max-log or exact log-sum-exp;
+//                     max-log or exact log-sum-exp: a unimodal Gray form
+//                     (one peak search, compile-time halves) for Gray
+//                     labels over equally spaced levels with a concave log
+//                     prior, a general form (argmax per half) otherwise;
 //   points            the metrics of all M points (any constellation).
 //
 // Semantics follow commkit/mapping/_numba_llr.py:35-235 and
@@ -20,7 +23,9 @@
 //     so an LLR near zero (x near a decision boundary) keeps its relative
 //     accuracy instead of cancelling two large metrics;
 //   - the exact sums take exp(m - m_peak) with that factored difference, one
-//     exponential per level (or point), and ocml's expf/logf.
+//     exponential per level (or point; the hardware exp2 in the Gray form,
+//     ocml's exp2f elsewhere), and the two log(1 + T) terms of a bit as one
+//     logf (log1p_diff).
 //
 // Layout: per axis, the tables (levels, log prior, labels, opposite levels,
 // bit positions) are staged in LDS once per workgroup.  Each lane writes its
