@@ -250,9 +250,9 @@ __device__ __forceinline__ double diff_to(double la, double pa, double xd, doubl
 }
 
 // Below this (in nats) under the peak, the other half's best level is summed
-// relative to itself: hardware exp2 of a term more than 126 octaves under the
-// peak returns zero, which against a best term above exp(-60) leaves out at
-// most exp(-27) of the half.
+// relative to itself.  Hardware exp2 may flush a term more than 126 octaves
+// under the peak to zero; against a best term above exp(-60) such a term is
+// under exp(-27) of the half, so flushing it changes nothing measurable.
 constexpr double kWeakGray = -60.0;
 
 // log(1 + a) - log(1 + b) for a, b >= 0 as one logarithm: oriented so that
