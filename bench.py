@@ -40,6 +40,9 @@ MARGIN_DB = 0.3
 # read inside this process, so the figure is reported only for the workload
 # it was measured on.
 PROFILED_TRAFFIC = {"frames": FRAMES, "bytes": 2.320e9}
+# The same for the sum-product decoder launch of the shaped workload (3106
+# codewords; profiles/ldpc_sum_product_shaped256_3106cw.md).
+PROFILED_TRAFFIC_SHAPED = {"frames": 3106, "bytes": 8.415e9}
 
 
 def _parse() -> argparse.Namespace:
@@ -512,7 +515,8 @@ def run_shaped(args: argparse.Namespace) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (PROFILED_TRAFFIC_SHAPED["bytes"] if F == PROFILED_TRAFFIC_SHAPED["frames"]
+                        and args.esn0 is None else None),
             "kernel": "ldpc_qc_layered<sum-product> (decode)",
             "bytes_per_launch": dec_bytes,
         },
