@@ -657,10 +657,25 @@ def device_code(code: LDPCCode, device: Any) -> DeviceCode:
     return DeviceCode(s, t)
 
 
-#: Tile choices per (n, lifting, rule, schedule) written by the autotuner
-#: (commkit_amd.tune, file tuned.json of the package); absent keys use the
-#: library's defaults.
-TUNED: dict[tuple[int, int, int, int], tuple[int, int, int, int]] = {}
+def code_fingerprint(code: LDPCCode) -> str:
+    """A digest of the parity-check matrix (sizes, puncturing and every edge):
+    two codes share it only when they are the same code."""
+    import hashlib
+
+    h = hashlib.sha256()
+    h.update(np.asarray([code.num_checks, code.num_variables, code.punctured], "<i8").tobytes())
+    h.update(np.ascontiguousarray(code.checks, "<i4").tobytes())
+    h.update(np.ascontiguousarray(code.variables, "<i4").tobytes())
+    return h.hexdigest()[:16]
+
+
+#: Tile choices per (code fingerprint, rule, schedule) written by the
+#: autotuner (commkit_amd.tune, file tuned.json of the package).  A tile
+#: tuned for one code never applies to another, and one the library cannot
+#: run (LDS, row degree) falls back to the default tile; codes without an
+#: entry use the default.
+TUNED: dict[tuple[str, int, int], tuple[int, int, int, int]] = {}
+_DEFAULT_TILES = (0, 0, 0, -1)
 
 
 def _load_tuned() -> None:
@@ -671,11 +686,28 @@ def _load_tuned() -> None:
     if not path.exists():
         return
     for e in json.loads(path.read_text()):
-        TUNED[(int(e["n"]), int(e["lifting"]), int(e["rule"]), int(e["schedule"]))] = tuple(
+        if "fingerprint" not in e:
+            continue  # an entry without the code's fingerprint applies to no code
+        TUNED[(str(e["fingerprint"]), int(e["rule"]), int(e["schedule"]))] = tuple(
             int(v) for v in e["tiles"])  # type: ignore[assignment]
 
 
 _load_tuned()
+
+
+def _fingerprint_cached(code: LDPCCode) -> str:
+    fp = code.__dict__.get("_fingerprint")
+    if fp is None:
+        fp = code_fingerprint(code)
+        code.__dict__["_fingerprint"] = fp
+    return fp
+
+
+def tuned_tiles(code: LDPCCode, rule: int, schedule: int) -> tuple[int, int, int, int] | None:
+    """The autotuner's tile for this exact code, rule and schedule, if any."""
+    if not TUNED:
+        return None
+    return TUNED.get((_fingerprint_cached(code), rule, schedule))
 
 
 def method_struct(code: LDPCCode, method: MinSum | SumProduct) -> CkLdpcMethod:
@@ -687,9 +719,7 @@ def method_struct(code: LDPCCode, method: MinSum | SumProduct) -> CkLdpcMethod:
     m.offset = 0.0 if sp else float(np.float32(method.offset))  # type: ignore[union-attr]
     m.max_iterations = int(method.max_iterations)
     m.early_stop = int(bool(method.early_stop))
-    qc = code._qc_view
-    key = (code.num_variables, int(qc["lifting"]) if qc is not None else 0, m.rule, m.schedule)
-    kernel, frames, lanes, state = TUNED.get(key, (0, 0, 0, -1))
+    kernel, frames, lanes, state = tuned_tiles(code, m.rule, m.schedule) or _DEFAULT_TILES
     m.kernel, m.frames_per_group, m.lanes_per_row, m.state_in_lds = kernel, frames, lanes, state
     return m
 
@@ -714,6 +744,11 @@ def _decode(code: LDPCCode, llrs: Any, method: MinSum | SumProduct, *, tiles: An
         if tiles is not None:
             ms.kernel, ms.frames_per_group, ms.lanes_per_row, ms.state_in_lds = tiles
         per = lib.ck_ldpc_workspace_bytes(ctypes.byref(dc.struct), ctypes.byref(ms), 1)
+        if per < 0 and tiles is None and (ms.kernel, ms.frames_per_group, ms.lanes_per_row,
+                                          ms.state_in_lds) != _DEFAULT_TILES:
+            # a tuned tile this device cannot run: the default tile instead
+            ms.kernel, ms.frames_per_group, ms.lanes_per_row, ms.state_in_lds = _DEFAULT_TILES
+            per = lib.ck_ldpc_workspace_bytes(ctypes.byref(dc.struct), ctypes.byref(ms), 1)
         if per < 0:
             _native.check(lib.ck_ldpc_decode(ctypes.byref(dc.struct), ctypes.byref(ms),
                                              None, None, None, None, None, 0, None, None))  # fmt: skip

@@ -36,11 +36,6 @@ struct alignas(16) CheckState {
     int idx;
 };
 
-__device__ __forceinline__ float phi(float x) {
-    x = fminf(fmaxf(x, kPhiMin), kPhiMax);
-    return -logf(tanhf(0.5f * x));
-}
-
 // Sign bitmaps of the syndrome check: per frame and block column j, bit c
 // is (APP of variable jZ + (c mod Z)) < 0 for c in [0, 64 ceil((Z + 32) / 64)),
 // so the 32 rows from any row below Z read two adjacent words, unwrapped.
@@ -309,6 +304,13 @@ __device__ __forceinline__ float& lds_at(float* ar, int off) {
 __device__ __forceinline__ int lane_xor(int v, int off) {
     return off == 1 ? __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false)   // quad_perm 1,0,3,2
                     : __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+}
+
+// The value of lane J of the quad in every lane of it (quad_perm J,J,J,J);
+// the quad's lanes must be active, as for lane_xor.
+template <int J>
+__device__ __forceinline__ int quad_lane(int v) {
+    return __builtin_amdgcn_mov_dpp(v, J * 0x55, 0xF, 0xF, false);
 }
 
 // Both values of a two-lane check row (lanes i and i + 32) in each of its
@@ -643,9 +645,10 @@ __device__ __forceinline__ float phi_fast(float x) {
 // old messages of the lane's edges are loaded together up front (slots past
 // the row's end load a real slot of the row and are ignored); Q, phi(|Q|)
 // and the LDS address of every edge stay in registers between the passes.
-// With P lanes a row's phi sum and sign parity are combined across the
-// lanes (the sum in a different order than one lane's, within the
-// tolerance of sum-product).
+// The phi sum is taken in ascending edge order whatever P is: with P lanes
+// every lane gathers the row's phi values of each slot (lane p holds edge
+// tP + p) and adds them in edge order, so all lanes hold the one-lane sum
+// and every tile decodes bit for bit alike.  Padding edges add +0.
 template <int P, int NT, bool UPDATE_APP>
 __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __restrict__ Rm,
                                             int s0, int dc, int r, int p, int Z, bool first) {
@@ -672,20 +675,33 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
         const bool valid = (t < NT - S) || (e < dc);
         q[t] = *addr[t] - rold[t];
         ph[t] = valid ? phi_fast(fabsf(q[t])) : 0.0f;
-        total = total + ph[t];
         neg ^= (valid && q[t] < 0.0f) ? 1u : 0u;
     }
-    if constexpr (P == 2) {
-        const int2 vt = row_pair(__float_as_int(total));
+    if constexpr (P == 1) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) total = total + ph[t];
+    } else if constexpr (P == 2) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int2 v = row_pair(__float_as_int(ph[t]));  // edges 2t, 2t + 1
+            total = total + __int_as_float(v.x);
+            total = total + __int_as_float(v.y);
+        }
         const int2 vn = row_pair(static_cast<int>(neg));
-        total = __int_as_float(vt.x) + __int_as_float(vt.y);
         neg = static_cast<unsigned>(vn.x ^ vn.y);
     } else {
+        static_assert(P == 4, "lanes per row: 1, 2 or 4");
 #pragma unroll
-        for (int off = 1; off < P; off <<= 1) {
-            total = total + __int_as_float(lane_xor(__float_as_int(total), off));
-            neg ^= static_cast<unsigned>(lane_xor(static_cast<int>(neg), off));
+        for (int t = 0; t < NT; ++t) {  // edges 4t .. 4t + 3
+            const int v = __float_as_int(ph[t]);
+            total = total + __int_as_float(quad_lane<0>(v));
+            total = total + __int_as_float(quad_lane<1>(v));
+            total = total + __int_as_float(quad_lane<2>(v));
+            total = total + __int_as_float(quad_lane<3>(v));
         }
+#pragma unroll
+        for (int off = 1; off < P; off <<= 1)
+            neg ^= static_cast<unsigned>(lane_xor(static_cast<int>(neg), off));
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {

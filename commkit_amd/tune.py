@@ -1,16 +1,18 @@
 """Tile autotuner of the quasi-cyclic LDPC decoder.
 
 The QC kernels take a tile: frames per workgroup ``F``, lanes per check row
-``P`` and whether the min-sum check state lives in LDS.  For min-sum every
-tile decodes bit for bit the same (the tests hold each to the oracle); for
-sum-product, P changes the order in which a row's phi terms are added
-(results within the sum-product tolerance).  Tiles differ otherwise only
-in speed, which depends on the code (lifting, row degrees, APP size against
-LDS) and on the batch.  :func:`tune` times the candidate tiles on given LLRs
-and checks that they agree; ``python -m commkit_amd.tune`` tunes the seeded
-codes of the benchmark workloads and writes ``tuned.json`` beside this file,
-which :mod:`commkit_amd.coding.ldpc` loads at import (key: total variables,
-lifting, rule, schedule).
+``P`` and whether the min-sum check state lives in LDS.  Every tile decodes
+bit for bit the same, for min-sum and for sum-product alike (the lanes of a
+row add its phi terms in ascending edge order whatever ``P`` is): APP LLRs,
+hard bits, iteration counts and valid flags.  Tiles differ only in speed,
+which depends on the code (lifting, row degrees, APP size against LDS) and
+on the batch.  :func:`tune` times the candidate tiles on given LLRs and
+rejects any tile whose outputs differ from the first one's in a single
+value; ``python -m commkit_amd.tune`` tunes the seeded codes of the
+benchmark workloads and writes ``tuned.json`` beside this file, which
+:mod:`commkit_amd.coding.ldpc` loads at import, keyed by the code's
+fingerprint, rule and schedule (a tile is a hint: one the library cannot
+run for a code falls back to its default).
 """
 
 from __future__ import annotations
@@ -48,17 +50,12 @@ def candidates(code: Any, method: Any) -> list[tuple[int, int, int, int]]:
     return out
 
 
-def _agree(out: Any, ref: Any, sum_product: bool) -> bool:
-    """Min-sum tiles agree bit for bit; sum-product tiles with several lanes
-    per row add a row's phi terms in another order, so they agree within the
-    sum-product tolerance (rtol 1e-3, atol 1e-3) and on nearly every bit."""
+def _agree(out: Any, ref: Any) -> bool:
+    """Two tiles agree when bits, APP LLRs, valid flags and iteration counts
+    are equal element for element (for min-sum and sum-product alike)."""
     import torch
 
-    if not sum_product:
-        return all(torch.equal(a, b) for a, b in zip(out, ref))
-    bits_off = float((out[0] != ref[0]).float().mean())
-    return bits_off < 1e-4 and bool(torch.allclose(out[1], ref[1], rtol=1e-3, atol=1e-3) or
-                                    bits_off == 0.0)
+    return all(torch.equal(a, b) for a, b in zip(out, ref))
 
 
 def tune(code: Any, method: Any, llrs: Any, *, repeats: int = 3,
@@ -67,9 +64,7 @@ def tune(code: Any, method: Any, llrs: Any, *, repeats: int = 3,
     fastest tile and the table of timings.  Raises if two tiles disagree."""
     import torch
 
-    from .coding.ldpc import SumProduct, _decode
-
-    sp = isinstance(method, SumProduct)
+    from .coding.ldpc import _decode
 
     rows = []
     ref = None
@@ -82,7 +77,7 @@ def tune(code: Any, method: Any, llrs: Any, *, repeats: int = 3,
             continue
         if ref is None:
             ref = out
-        elif not _agree(out, ref, sp):
+        elif not _agree(out, ref):
             raise RuntimeError(f"tile {tiles} disagrees with {rows[0]['tiles']}")
         times = []
         for _ in range(repeats):
@@ -122,7 +117,7 @@ def _workloads():
 
 
 def main() -> None:
-    from .coding.ldpc import _SCHEDULES
+    from .coding.ldpc import _SCHEDULES, code_fingerprint
     from .mapping.constellation import Constellation
     from .mapping.demap import llr_any
     from .operating import waterfall_db
@@ -138,7 +133,8 @@ def main() -> None:
         best, rows = tune(code, method, llrs, verbose=True)
         sp = type(method).__name__ == "SumProduct"
         entries.append({
-            "code": name, "n": code.num_variables, "lifting": int(code._qc_view["lifting"]),
+            "code": name, "fingerprint": code_fingerprint(code), "n": code.num_variables,
+            "lifting": int(code._qc_view["lifting"]),
             "rule": int(sp), "schedule": _SCHEDULES.index(method.schedule),
             "frames": frames, "tiles": list(best), "ms": min(r["ms"] for r in rows),
         })  # fmt: skip

@@ -16,6 +16,10 @@
  *                            (phi, min_sum_row, sum_product_row, satisfied,
  *                            ldpc_layered): rows in the given schedule order,
  *                            parallel over frames.
+ *   oracle_ldpc_layered_probe  the layered sum-product with phi evaluated in
+ *                            float64 and rounded once (phi_probe): not the
+ *                            reference; it measures the float32 algorithm's
+ *                            sensitivity to the rounding of phi.
  *   oracle_ldpc_flooding     no reference counterpart (the reference has only
  *                            the layered schedule).  Restates this library's
  *                            flooding definition: each check updates from
@@ -53,6 +57,20 @@ static float phi(float x) {
     return -logf(tanhf(0.5f * x));
 }
 
+/* The conditioning probe's phi: the same clipped function evaluated in
+ * float64 as log((1 + e^-x) / (1 - e^-x)) and rounded once to float32, so it
+ * differs from phi() only by float32 rounding (a fraction of an ulp to a few
+ * ulps).  Not the reference's arithmetic: decoding with it measures how far a
+ * float32 sum-product decoder moves when phi is rounded differently. */
+static float phi_probe(float x) {
+    if (x < 1e-7f) x = 1e-7f;
+    if (x > 30.0f) x = 30.0f;
+    const double d = (double)x;
+    return (float)log((1.0 + exp(-d)) / -expm1(-d));
+}
+
+typedef float (*phi_fn)(float);
+
 static void min_sum_row(float* a, float* r, const int32_t* cols, int e0, int e1, float scale,
                         float offset) {
     /* _numba_loops.py:43-68 */
@@ -87,7 +105,7 @@ static void min_sum_row(float* a, float* r, const int32_t* cols, int e0, int e1,
     }
 }
 
-static void sum_product_row(float* a, float* r, const int32_t* cols, int e0, int e1) {
+static void sum_product_row(float* a, float* r, const int32_t* cols, int e0, int e1, phi_fn f) {
     /* _numba_loops.py:71-94: pass 1 keeps Q in a and phi(|Q|) in r */
     float total = 0.0f;
     int neg = 0;
@@ -95,7 +113,7 @@ static void sum_product_row(float* a, float* r, const int32_t* cols, int e0, int
         int c = cols[e];
         float q = a[c] - r[e];
         neg ^= (q < 0.0f);
-        float p = phi(fabsf(q));
+        float p = f(fabsf(q));
         total += p;
         a[c] = q;
         r[e] = p;
@@ -103,7 +121,7 @@ static void sum_product_row(float* a, float* r, const int32_t* cols, int e0, int
     for (int e = e0; e < e1; ++e) {
         int c = cols[e];
         float q = a[c];
-        float mag = phi(total - r[e]);
+        float mag = f(total - r[e]);
         float rn = (neg ^ (q < 0.0f)) ? -mag : mag;
         a[c] = q + rn;
         r[e] = rn;
@@ -120,12 +138,34 @@ static int satisfied(const float* a, const int32_t* row_ptr, const int32_t* cols
     return 1;
 }
 
+static void layered(float* app, int64_t R, int32_t N, const int32_t* row_ptr, const int32_t* cols,
+                    int32_t m, int32_t sum_product, float scale, float offset,
+                    int32_t max_iterations, int32_t early_stop, int32_t* iters, uint8_t* valid,
+                    phi_fn f);
+
 /* Layered decoding of R frames of app (R, N) in place; rows in schedule
  * order (row_ptr, cols) as LDPCCode._schedule gives them. */
 void oracle_ldpc_layered(float* app, int64_t R, int32_t N, const int32_t* row_ptr,
                          const int32_t* cols, int32_t m, int32_t sum_product, float scale,
                          float offset, int32_t max_iterations, int32_t early_stop,
                          int32_t* iters, uint8_t* valid) {
+    layered(app, R, N, row_ptr, cols, m, sum_product, scale, offset, max_iterations, early_stop,
+            iters, valid, phi);
+}
+
+/* Layered sum-product with phi_probe in place of phi: the conditioning
+ * probe (no reference counterpart; test infrastructure). */
+void oracle_ldpc_layered_probe(float* app, int64_t R, int32_t N, const int32_t* row_ptr,
+                               const int32_t* cols, int32_t m, int32_t max_iterations,
+                               int32_t early_stop, int32_t* iters, uint8_t* valid) {
+    layered(app, R, N, row_ptr, cols, m, 1, 1.0f, 0.0f, max_iterations, early_stop, iters, valid,
+            phi_probe);
+}
+
+static void layered(float* app, int64_t R, int32_t N, const int32_t* row_ptr, const int32_t* cols,
+                    int32_t m, int32_t sum_product, float scale, float offset,
+                    int32_t max_iterations, int32_t early_stop, int32_t* iters, uint8_t* valid,
+                    phi_fn f) {
     const int E = row_ptr[m];
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t b = 0; b < R; ++b) {
@@ -136,7 +176,7 @@ void oracle_ldpc_layered(float* app, int64_t R, int32_t N, const int32_t* row_pt
         while (!ok && it < max_iterations) {
             for (int row = 0; row < m; ++row) {
                 if (sum_product)
-                    sum_product_row(a, r, cols, row_ptr[row], row_ptr[row + 1]);
+                    sum_product_row(a, r, cols, row_ptr[row], row_ptr[row + 1], f);
                 else
                     min_sum_row(a, r, cols, row_ptr[row], row_ptr[row + 1], scale, offset);
             }

@@ -23,6 +23,7 @@ _f32 = ctypes.c_float
 
 _SIGS = {
     "oracle_ldpc_layered": [_p, _i64, _i32, _p, _p, _i32, _i32, _f32, _f32, _i32, _i32, _p, _p],
+    "oracle_ldpc_layered_probe": [_p, _i64, _i32, _p, _p, _i32, _i32, _i32, _p, _p],
     "oracle_ldpc_flooding": [_p, _i64, _i32, _p, _p, _i32, _p, _p, _i32, _f32, _f32, _i32, _i32,
                              _p, _p],
     "oracle_llr_points": [_p, _p, _i64, _p, _p, _p, _p, _i32, _i32, _f32, _i32, _p],
@@ -83,6 +84,22 @@ def ldpc_layered(app: np.ndarray, row_ptr, cols, *, sum_product: bool, scale: fl
         float(np.float32(scale)), float(np.float32(offset)), max_iterations, int(early_stop),
         _ptr(iters), _ptr(valid),
     )  # fmt: skip
+    return app, iters, valid.astype(bool)
+
+
+def ldpc_layered_probe(app: np.ndarray, row_ptr, cols, *, max_iterations: int,
+                       early_stop: bool):
+    """Layered sum-product as :func:`ldpc_layered` but with phi evaluated in
+    float64 and rounded once to float32: a probe of how far the float32
+    algorithm moves under a change of phi's rounding alone (not the
+    reference's arithmetic)."""
+    app = np.ascontiguousarray(app, dtype=np.float32)
+    row_ptr, cols = _i32a(row_ptr), _i32a(cols)
+    R, N = app.shape
+    iters = np.empty(R, np.int32)
+    valid = np.empty(R, np.uint8)
+    lib().oracle_ldpc_layered_probe(_ptr(app), R, N, _ptr(row_ptr), _ptr(cols), row_ptr.size - 1,
+                                    max_iterations, int(early_stop), _ptr(iters), _ptr(valid))  # fmt: skip
     return app, iters, valid.astype(bool)
 
 

@@ -227,3 +227,19 @@ def test_shaped_pas_frames_decode(gpu):
     np.testing.assert_array_equal(bits, words[:, : code.k].cpu().numpy())  # systematic
     ccdm = fr.pas.matcher
     assert torch.equal(words[:, ccdm.n : code.k], fr.info[:, ccdm.k :])  # payload on signs
+
+
+def test_infeasible_tuned_tile_falls_back(gpu, monkeypatch):
+    """A tuned tile the library cannot run for a code (here the check state
+    in LDS beside two 72 KB APPs) is a hint: decode takes the default tile
+    and gives the same results; the same tile asked for explicitly fails."""
+    code = seeded_code("r4/5-z360")
+    key = (ldpc_module.code_fingerprint(code), 0, 0)
+    monkeypatch.setitem(ldpc_module.TUNED, key, (2, 2, 1, 1))
+    llrs = _llrs(code, 3, 1.6, seed=11)
+    method = MinSum(max_iterations=8)
+    assert_identical(gpu_decode(code, llrs, method), gpu_decode(code, llrs, method, (0, 0, 0, -1)),
+                     "fallback")  # fmt: skip
+    assert_identical(gpu_decode(code, llrs, method), oracle_decode(code, llrs, method), "oracle")
+    with pytest.raises(RuntimeError, match="LDS"):
+        gpu_decode(code, llrs, method, (2, 2, 1, 1))

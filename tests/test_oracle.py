@@ -146,6 +146,42 @@ def test_layered_matches_textbook(sp, scale, offset):
         np.testing.assert_allclose(got[b], ref, atol=5e-3 if sp else 1e-4, rtol=1e-4)
 
 
+def test_phi_probe_is_a_rounding_change_only():
+    """The conditioning probe decodes sum-product with phi evaluated in
+    float64: on small LLRs (no saturated message) it agrees with the oracle
+    and with the float64 textbook as closely as the oracle does; where a
+    row's phi total is dominated by one edge, both resolve the other edges'
+    message only to about ln 2 (one float32 ulp of the total), the limit the
+    GPU sum-product tests measure against."""
+    code = small_qc()
+    row_ptr, cols, _ = code._schedule
+    llrs = _llrs(code.n, 3, 1.0, 3)
+    ora, _, _ = cpu.ldpc_layered(llrs.copy(), row_ptr, cols, sum_product=True, max_iterations=3,
+                                 early_stop=False)  # fmt: skip
+    pro, _, _ = cpu.ldpc_layered_probe(llrs.copy(), row_ptr, cols, max_iterations=3,
+                                       early_stop=False)  # fmt: skip
+    np.testing.assert_allclose(pro, ora, rtol=1e-5, atol=1e-5)
+    for b in range(3):
+        ref = layered_fp64(llrs[b], rows_in_schedule(code), sum_product=True, iterations=3)
+        np.testing.assert_allclose(pro[b], ref, atol=5e-3, rtol=1e-4)
+    # A check of two edges, one reliable (s) and one weak (0.25): R to the
+    # reliable edge is phi(phi(0.25)) = 0.25 to float32 rounding, while R to
+    # the weak edge is phi(total - phi(0.25)) = phi(phi(s)) of a difference
+    # resolved only to ulps of total ~ 2.08, so it misses s by up to ln 2
+    # once phi(s) ~ 2 e^-s is that small.
+    rp = np.array([0, 2], np.int32)
+    c2 = np.array([0, 1], np.int32)
+    for strong in (12.0, 14.0, 16.0):
+        x = np.array([[strong, 0.25]], np.float32)
+        for fn in (lambda a: cpu.ldpc_layered(a, rp, c2, sum_product=True, max_iterations=1,
+                                              early_stop=False),
+                   lambda a: cpu.ldpc_layered_probe(a, rp, c2, max_iterations=1,
+                                                    early_stop=False)):  # fmt: skip
+            app, _, _ = fn(x.copy())
+            assert abs(app[0, 0] - (strong + 0.25)) <= 1e-5 * strong
+            assert abs(app[0, 1] - (strong + 0.25)) <= np.log(2.0)
+
+
 @pytest.mark.parametrize(
     "sp,scale,offset", [(False, 0.75, 0.1), (True, 1.0, 0.0)], ids=["min-sum", "sum-product"]
 )
