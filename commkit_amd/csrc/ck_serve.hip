@@ -10,21 +10,26 @@
 //                             code's ring (CK_EBUSY when the ring, i.e. the
 //                             budget, is full; the caller retries later);
 //   submit(tag)               the caller has written the request's LLRs;
-//   step()                    one launch over every submitted request not
-//                             yet launched (contiguous slots of one code
-//                             merge into one class; at most 8 classes, the
-//                             rest wait), if a launch workspace is free;
+//   step()                    one launch over the submitted requests not yet
+//                             launched, oldest submission first, up to
+//                             max_launch_frames frames (at least one
+//                             request; 0 = no cap) and 8 classes
+//                             (contiguous slots of one code merge into one
+//                             class), if a launch workspace is free;
 //   poll()                    requests whose launch has completed;
 //   release(tag)              the caller has read the results; the slots
 //                             return to the ring in admission order.
 //
 // Requests therefore join the next launch as soon as they arrive instead of
 // waiting for a full batch, and a launch in flight never blocks admission
-// or submission.  Each workspace has its own stream, so launches in flight
+// or submission.  The frame cap bounds how long one launch holds its
+// requests: a burst is launched in pieces, so a request arriving behind it
+// joins a launch of bounded size instead of waiting for the whole burst.  Each workspace has its own stream, so launches in flight
 // overlap on the device (a new launch's workgroups take the CUs that the
 // previous one's finished workgroups free); a launch first waits for the
 // work enqueued on the caller's stream (the requests' LLRs), and records an
-// event that poll() queries without blocking.  A frame decodes the same
+// event that poll() queries without blocking; a launch completes as soon as
+// its event has fired, whether or not older launches still run.  A frame decodes the same
 // alone or in any launch (the batched decoder's contract), so a request's
 // results do not depend on the traffic it was served with.
 //
@@ -52,6 +57,7 @@ struct Request {
     long long frames;
     int state;
     int launch;        // workspace slot of its launch
+    long long order;   // submission order
 };
 
 struct Ring {
@@ -81,6 +87,8 @@ struct ck_serve {
     hipEvent_t inputs = nullptr;      // recorded there at each launch, waited on by the launch
     bool dry = false;
     long long next_seq = 0;
+    long long next_order = 0;
+    long long max_launch_frames = 0;  // 0: no cap
     int64_t workspace_bytes = 0;
     std::vector<uint64_t> completed;  // tags completed, not yet returned by poll
 };
@@ -105,6 +113,8 @@ int32_t check_config(const ck_serve_config* cfg, bool need_buffers = true) {
                cfg->max_inflight);
     if (int32_t st = ck::check_method(cfg->method)) return st;
     CK_REQUIRE(cfg->method->schedule == 0, "serving runs the layered schedule");
+    CK_REQUIRE(cfg->max_launch_frames >= 0, "max_launch_frames %lld < 0",
+               static_cast<long long>(cfg->max_launch_frames));
     for (int i = 0; i < cfg->num_codes; ++i) {
         CK_REQUIRE(cfg->codes[i] != nullptr, "code %d is NULL", i);
         if (int32_t st = ck::check_code(cfg->codes[i])) return st;
@@ -137,27 +147,29 @@ int32_t workspace_bound(const ck_serve_config* cfg, int64_t* out) {
     return CK_OK;
 }
 
-// Completes, in launch order, every launch whose event has fired.
+// Completes every launch whose event has fired, oldest first (a short
+// launch started after a long one completes without waiting for it).
 int32_t reap(ck_serve* s) {
-    while (true) {
-        Launch* oldest = nullptr;
-        for (Launch& l : s->launches)
-            if (l.busy && (oldest == nullptr || l.seq < oldest->seq)) oldest = &l;
-        if (oldest == nullptr) return CK_OK;
+    std::vector<Launch*> busy;
+    for (Launch& l : s->launches)
+        if (l.busy) busy.push_back(&l);
+    std::sort(busy.begin(), busy.end(), [](const Launch* a, const Launch* b) { return a->seq < b->seq; });
+    for (Launch* l : busy) {
         if (!s->dry) {
-            const hipError_t e = hipEventQuery(oldest->done);
-            if (e == hipErrorNotReady) return CK_OK;
+            const hipError_t e = hipEventQuery(l->done);
+            if (e == hipErrorNotReady) continue;
             if (e != hipSuccess) return ck::hip_status(e, "hipEventQuery");
         }
-        const int w = static_cast<int>(oldest - s->launches.data());
+        const int w = static_cast<int>(l - s->launches.data());
         for (Ring& r : s->rings)
             for (Request& q : r.live)
                 if (q.state == kLaunched && q.launch == w) {
                     q.state = kDone;
                     s->completed.push_back(q.tag);
                 }
-        oldest->busy = false;
+        l->busy = false;
     }
+    return CK_OK;
 }
 
 }  // namespace
@@ -183,6 +195,7 @@ extern "C" int32_t ck_serve_create(const ck_serve_config* config, void* const* w
     s->method = *config->method;
     s->stream = static_cast<hipStream_t>(stream);
     s->dry = config->dry_run != 0;
+    s->max_launch_frames = config->max_launch_frames;
     s->workspace_bytes = ws;
     for (int i = 0; i < config->num_codes; ++i) {
         s->codes.push_back(*config->codes[i]);
@@ -251,10 +264,10 @@ extern "C" int32_t ck_serve_admit(ck_serve* s, int32_t code, int64_t frames, uin
         return ck::fail(CK_EBUSY, "code %d: %lld frames do not fit (%lld of %lld in use)", code,
                         static_cast<long long>(frames), r.tail - r.head, cap);
     if (gap > 0) {
-        r.live.push_back(Request{0, code, r.tail, gap, kGap, -1});
+        r.live.push_back(Request{0, code, r.tail, gap, kGap, -1, -1});
         r.tail += gap;
     }
-    r.live.push_back(Request{tag, code, r.tail, frames, kAdmitted, -1});
+    r.live.push_back(Request{tag, code, r.tail, frames, kAdmitted, -1, -1});
     r.tail += frames;
     const ck_ldpc_code& c = s->codes[static_cast<size_t>(code)];
     where->first = pos;
@@ -273,6 +286,7 @@ extern "C" int32_t ck_serve_submit(ck_serve* s, uint64_t tag) {
     CK_REQUIRE(q->state == kAdmitted, "tag %llu was already submitted",
                static_cast<unsigned long long>(tag));
     q->state = kSubmitted;
+    q->order = s->next_order++;
     return CK_OK;
 }
 
@@ -287,7 +301,16 @@ extern "C" int32_t ck_serve_step(ck_serve* s, int64_t* launched_frames) {
             break;
         }
     if (w < 0) return CK_OK;  // every workspace in flight: the next step launches
-    // Runs of submitted requests, contiguous in their ring, become classes.
+    // The requests of this launch: submitted ones in submission order, up to
+    // the frame cap (at least one) and kMaxClasses runs.
+    std::vector<Request*> pending;
+    for (Ring& r : s->rings)
+        for (Request& q : r.live)
+            if (q.state == kSubmitted) pending.push_back(&q);
+    if (pending.empty()) return CK_OK;
+    std::sort(pending.begin(), pending.end(),
+              [](const Request* a, const Request* b) { return a->order < b->order; });
+    // Runs of chosen requests, contiguous in their ring, become classes.
     struct Run {
         int code;
         long long pos;
@@ -295,31 +318,27 @@ extern "C" int32_t ck_serve_step(ck_serve* s, int64_t* launched_frames) {
         std::vector<Request*> reqs;
     };
     std::vector<Run> runs;
-    for (size_t ci = 0; ci < s->rings.size(); ++ci) {
-        Ring& r = s->rings[ci];
-        const long long cap = r.buf.capacity;
-        Run* cur = nullptr;
-        for (Request& q : r.live) {
-            if (q.state != kSubmitted) {
-                cur = nullptr;
-                continue;
-            }
-            const long long pos = q.first % cap;
-            if (cur != nullptr && cur->pos + cur->frames == pos) {
-                cur->frames += q.frames;
-                cur->reqs.push_back(&q);
-            } else {
-                runs.push_back(Run{static_cast<int>(ci), pos, q.frames, {&q}});
-                cur = &runs.back();
-            }
+    long long chosen = 0;
+    for (Request* q : pending) {
+        if (s->max_launch_frames > 0 && chosen > 0 && chosen + q->frames > s->max_launch_frames)
+            break;
+        const long long pos = q->first % s->rings[static_cast<size_t>(q->code)].buf.capacity;
+        Run* ext = nullptr;
+        for (Run& run : runs)
+            if (run.code == q->code && run.pos + run.frames == pos) ext = &run;
+        if (ext != nullptr) {
+            ext->frames += q->frames;
+            ext->reqs.push_back(q);
+        } else {
+            if (runs.size() == static_cast<size_t>(ckbatch::kMaxClasses)) break;
+            runs.push_back(Run{q->code, pos, q->frames, {q}});
         }
+        chosen += q->frames;
     }
-    if (runs.empty()) return CK_OK;
-    // Largest frames first, then oldest; at most kMaxClasses classes.
+    // Largest frames first (the batched decoder's cost order), then oldest.
     std::stable_sort(runs.begin(), runs.end(), [&](const Run& a, const Run& b) {
         return s->codes[static_cast<size_t>(a.code)].n > s->codes[static_cast<size_t>(b.code)].n;
     });
-    if (runs.size() > static_cast<size_t>(ckbatch::kMaxClasses)) runs.resize(ckbatch::kMaxClasses);
     ck_batch_class cls[ckbatch::kMaxClasses];
     std::memset(cls, 0, sizeof(cls));
     long long total = 0;

@@ -307,3 +307,49 @@ def llr_rows(c: "Constellation", x: Any, noise_var: Any, method: str, out: Any =
             int(closed), int(method == "exact"), out.data_ptr(), _native.stream_of(x.device),
         ))  # fmt: skip
     return out
+
+
+class RowsDemapper:
+    """Per-row-noise demapping of one separable constellation, bound once.
+
+    For request loops that demap many small batches (a serving loop filling
+    ring slots): the device tables, the ``ck_axis`` structs and the stream
+    are resolved at construction, so a call is one checked C-ABI call.
+    ``demapper(x, inv_s2, out)`` writes the LLRs of ``x`` (``(R, S)``
+    complex64, contiguous) with row ``r``'s ``1 / sigma^2`` from ``inv_s2``
+    (``(R,)`` float64, contiguous) into ``out`` (``(R, S * k)`` float32,
+    contiguous), all on the constructor's device, on the stream current at
+    construction; the same LLRs as :func:`llr_rows`, bit for bit.
+    """
+
+    def __init__(self, c: "Constellation", method: str, device: Any) -> None:
+        import torch
+
+        check_method(method)
+        if c._axes is None:
+            raise ValueError("RowsDemapper: the constellation is not separable per axis")
+        self.device = torch.device(device)
+        with torch.cuda.device(self.device):
+            self._re, self._im, self._keep = axis_tables(c, self.device)
+            self._stream = _native.stream_of(self.device)
+        self._closed = int(closed_form(c._axes, method))
+        self._exact = int(method == "exact")
+        self.k = c.bits_per_symbol
+        self._lib = _native.lib()
+        self._dtypes = (torch.complex64, torch.float64, torch.float32)
+
+    def __call__(self, x: Any, inv_s2: Any, out: Any) -> Any:
+        R, S = int(x.shape[0]), int(x.shape[1])
+        cx, f64, f32 = self._dtypes
+        if (x.dtype != cx or not x.is_contiguous() or inv_s2.dtype != f64
+                or inv_s2.numel() != R or not inv_s2.is_contiguous() or out.dtype != f32
+                or tuple(out.shape) != (R, S * self.k) or not out.is_contiguous()
+                or x.device != self.device or inv_s2.device != self.device
+                or out.device != self.device):
+            raise ValueError("RowsDemapper: x (R, S) complex64, inv_s2 (R,) float64 and out "
+                             "(R, S * k) float32, contiguous, on the demapper's device")  # fmt: skip
+        if R * S:
+            _native.check(self._lib.ck_llr_separable_rows(
+                x.data_ptr(), R, S, ctypes.byref(self._re), ctypes.byref(self._im),
+                inv_s2.data_ptr(), self._closed, self._exact, out.data_ptr(), self._stream))  # fmt: skip
+        return out

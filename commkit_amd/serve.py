@@ -53,7 +53,8 @@ class CkServeRing(ctypes.Structure):
 class CkServeConfig(ctypes.Structure):
     _fields_ = [("codes", ctypes.c_void_p), ("rings", ctypes.c_void_p),
                 ("num_codes", ctypes.c_int32), ("max_inflight", ctypes.c_int32),
-                ("method", ctypes.c_void_p), ("dry_run", ctypes.c_int32)]
+                ("method", ctypes.c_void_p), ("dry_run", ctypes.c_int32),
+                ("max_launch_frames", ctypes.c_int64)]
 
 
 class CkServeSlot(ctypes.Structure):
@@ -106,11 +107,16 @@ class Server:
         only (for tests without a GPU).
     capacities : sequence of int, optional
         Frames per ring instead of the budget split.
+    max_launch_frames : int, optional
+        Frames one launch takes at most (whole requests, oldest first, at
+        least one request).  Bounds how long a burst holds the device before
+        a later request's launch; ``None`` or 0: no cap.
     """
 
     def __init__(self, codes: Sequence[LDPCCode], method: MinSum | SumProduct | None = None, *,
                  hbm_budget: int = 16 << 30, max_inflight: int = 2, device: Any = None,
-                 dry_run: bool = False, capacities: Sequence[int] | None = None) -> None:
+                 dry_run: bool = False, capacities: Sequence[int] | None = None,
+                 max_launch_frames: int | None = None) -> None:
         method = method or MinSum()
         if method.schedule != "layered":
             raise ValueError("serving runs the layered schedule")
@@ -139,8 +145,10 @@ class Server:
         self._ms = method_struct(self.codes[0], method)
         code_ptrs = (ctypes.c_void_p * len(codes))(*[ctypes.addressof(d.struct) for d in dev_codes])
         rings = (CkServeRing * len(codes))()
+        self.max_launch_frames = int(max_launch_frames or 0)
         cfg = CkServeConfig(ctypes.addressof(code_ptrs), ctypes.addressof(rings), len(codes),
-                            max_inflight, ctypes.addressof(self._ms), int(dry_run))  # fmt: skip
+                            max_inflight, ctypes.addressof(self._ms), int(dry_run),
+                            self.max_launch_frames)  # fmt: skip
         self._cfg, self._code_ptrs, self._rings = cfg, code_ptrs, rings
         # Workspace size first (rings only need capacities for it).
         for r in rings:
@@ -209,7 +217,8 @@ class Server:
         _native.check(self._lib.ck_serve_submit(self._h, int(tag)))
 
     def step(self) -> int:
-        """Launch every submitted request not yet launched; frames launched."""
+        """Launch the submitted requests not yet launched (oldest first, up to
+        the frame cap); frames launched."""
         n = ctypes.c_int64()
         _native.check(self._lib.ck_serve_step(self._h, ctypes.byref(n)))
         return int(n.value)

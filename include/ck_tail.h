@@ -294,6 +294,9 @@ typedef struct ck_serve_config {
     const ck_ldpc_method* method;      /* layered                         */
     int32_t dry_run;                   /* 1: no device work (host logic only;
                                           buffers may be NULL)            */
+    int64_t max_launch_frames;         /* frames one launch takes at most
+                                          (whole requests, at least one);
+                                          0 = no cap                      */
 } ck_serve_config;
 
 typedef struct ck_serve_slot {  /* where an admitted request's frames live */
@@ -323,9 +326,11 @@ int32_t ck_serve_admit(ck_serve* server, int32_t code, int64_t frames, uint64_t 
  * work, then run on streams of their own). */
 int32_t ck_serve_submit(ck_serve* server, uint64_t tag);
 
-/* Launches every submitted request not yet launched (up to 8 contiguous
- * runs) if a workspace is free; *launched_frames = frames launched (0:
- * nothing to launch, or every workspace in flight). */
+/* Launches the submitted requests not yet launched, oldest submission
+ * first, up to max_launch_frames frames and 8 contiguous runs, if a
+ * workspace is free; *launched_frames = frames launched (0: nothing to
+ * launch, or every workspace in flight).  A launch completes when its own
+ * work has finished, independently of launches started before it. */
 int32_t ck_serve_step(ck_serve* server, int64_t* launched_frames);
 
 /* Up to max_tags requests completed since the last poll (non-blocking). */

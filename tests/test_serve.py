@@ -104,3 +104,24 @@ def test_schedule_and_driver_with_fake_clock(codes):
     assert res.latency_ms.size == res.frames and np.all(res.latency_ms >= 0)
     assert res.rejected > 0  # 40-frame rings: bursts wait for room
     assert srv.usage(0)[0] == 0 and srv.usage(1)[0] == 0
+
+
+def test_launch_frame_cap_takes_oldest_whole_requests(codes):
+    """With a frame cap a launch takes whole requests in submission order
+    until the next one would exceed the cap (always at least one), so a
+    burst is launched in pieces; a request larger than the cap goes alone."""
+    s = Server(codes, MinSum(), dry_run=True, capacities=[64, 64], max_inflight=8,
+               max_launch_frames=10)  # fmt: skip
+    sizes = [(0, 4), (1, 5), (0, 3), (1, 12), (0, 2)]
+    for t, (code, f) in enumerate(sizes):
+        assert s.admit(code, f, t) is not None
+    for t in (1, 0, 2, 3, 4):  # submission order differs from admission order
+        s.submit(t)
+    assert s.step() == 9  # requests 1 and 0; request 2 would make 12
+    assert s.step() == 3  # request 2; request 3 would make 15
+    assert s.step() == 12  # request 3 alone, above the cap
+    assert s.step() == 2
+    assert s.step() == 0
+    assert sorted(s.poll()) == [0, 1, 2, 3, 4]
+    assert s.max_launch_frames == 10
+    s.close()
