@@ -6,15 +6,15 @@ lane, with the tables staged in LDS:
 - ``ck_llr_separable_maxlog``: max-log LLRs of a uniform separable
   lattice in closed form, per axis the snapped level against the nearer of
   the two levels just outside its run of equal bits
-  (``commkit/mapping/_demap.py:177-212``, ``commkit/_cuda/src/llr.cu:262-337``);
+  (``commkit/mapping/_demap.py:314-333``, ``commkit/_cuda/src/llr.cu:262-337``);
 - ``ck_llr_separable``: the metrics of each axis's levels, max-log or
-  exact log-sum-exp, with the per-axis log prior (``_demap.py:326-447``,
+  exact log-sum-exp, with the per-axis log prior (``_demap.py:335-352``,
   ``llr.cu:225-259``);
-- ``ck_llr_points``: the metrics of all M points (``_demap.py:245-314``,
+- ``ck_llr_points``: the metrics of all M points (``_demap.py:150-219``,
   ``llr.cu:200-223``).
 
 Routing follows ``Constellation.llr`` (``commkit/mapping/constellation.py:
-328-332``) and ``_closed_form`` (``_demap.py:207-212``): separable and
+328-332``) and ``_closed_form`` (``_demap.py:112-117``): separable and
 max-log without a prior takes the closed form, separable otherwise the level
 metrics, anything else the point metrics.  Sign convention: a positive LLR
 means bit 0 is more likely.
@@ -48,7 +48,7 @@ def noise_variance(
     real: bool,
     function_name: str,
 ) -> float:
-    """Noise variance from ``noise_var`` or ``esn0_db`` (``_demap.py:143-163``).
+    """Noise variance from ``noise_var`` or ``esn0_db`` (``_demap.py:48-68``).
 
     ``N0 = Es / 10^(EsN0/10)`` with ``Es`` the constellation's power; real
     symbols carry the in-phase half, ``N0 / 2``.
@@ -70,7 +70,7 @@ def check_method(method: str) -> None:
 
 
 def log_prior(pmf: np.ndarray | None, order: int) -> np.ndarray:
-    """float32 log prior, ``log(clip(pmf, 1e-40))``; zeros for uniform (``_demap.py:220-224``)."""
+    """float32 log prior, ``log(clip(pmf, 1e-40))``; zeros for uniform (``_demap.py:125-129``)."""
     if pmf is None:
         return np.zeros(order, dtype=np.float32)
     return np.log(np.clip(np.asarray(pmf, np.float64), 1e-40, None)).astype(np.float32)
@@ -82,7 +82,7 @@ def opposite_levels(labels: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     ``labels`` is ``(L, k)`` over ascending levels; returns int32 ``lo``,
     ``hi`` of shape ``(k, L)``, -1 where there is none.  These are the levels
     just outside the run of equal bits holding each level
-    (``_demap.py:177-197``).
+    (``_demap.py:82-106``).
     """
     L, k = labels.shape
     lo = np.full((k, L), -1, dtype=np.int32)

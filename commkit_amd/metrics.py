@@ -306,7 +306,7 @@ AWGN_NODES = 128
 def awgn_gmi(constellation: Any, *, esn0_db: Any) -> float | np.ndarray:
     """GMI (bit-metric decoding rate) of ``constellation`` over AWGN, in bits
     per symbol: ``H(X) - sum_b H(B_b | Y)`` with exact bit LLRs including the
-    prior (``commkit/metrics/awgn.py:477-525``).
+    prior (``commkit/metrics/awgn.py:477-523``).
 
     The expectation over the noise is Gauss-Hermite quadrature of order
     :data:`AWGN_NODES` per axis (a product rule in the plane for complex

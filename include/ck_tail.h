@@ -45,7 +45,7 @@ const char* ck_last_error(void);
 
 /* One axis of a separable square lattice (Gray QAM, shaped or not); all
  * pointers are device memory.  Reference: the per-axis tables built by
- * commkit/mapping/_demap.py:411-441 (levels, log prior, packed labels,
+ * commkit/mapping/_demap.py:316-347 (levels, log prior, packed labels,
  * opposite levels, bit positions).  Levels and prior are float64 here (the
  * reference uploads float32): the kernels form metric differences in
  * float64 from them, so an LLR keeps its relative accuracy down to zero. */
@@ -70,7 +70,7 @@ typedef struct ck_axis {
  * cost its rounding relative to the larger term).
  * Replaces the kernel `llr_separable_maxlog` (commkit/_cuda/src/llr.cu:
  * 304-337) reached through _cuda.get_kernel("llr_separable_maxlog")
- * (commkit/_cuda/llr.py:150; call site commkit/mapping/_demap.py:425). */
+ * (commkit/_cuda/llr.py:150; call site commkit/mapping/_demap.py:330). */
 int32_t ck_llr_separable_maxlog(const void* x, int64_t n, const ck_axis* re,
                                 const ck_axis* im, double inv_s2, float* out,
                                 void* stream);
@@ -78,7 +78,7 @@ int32_t ck_llr_separable_maxlog(const void* x, int64_t n, const ck_axis* re,
 /* LLRs from the metrics of each axis's levels, with the log prior:
  * exact = 0 max-log, 1 exact log-sum-exp.  Same layout as above.
  * Replaces `llr_separable` (commkit/_cuda/src/llr.cu:225-259; wrapper
- * commkit/_cuda/llr.py:87; call site commkit/mapping/_demap.py:444). */
+ * commkit/_cuda/llr.py:87; call site commkit/mapping/_demap.py:349). */
 int32_t ck_llr_separable(const void* x, int64_t n, const ck_axis* re,
                          const ck_axis* im, double inv_s2, int32_t exact,
                          float* out, void* stream);
@@ -115,7 +115,7 @@ int32_t ck_llr_separable_permuted(const void* x, int64_t n, const ck_axis* re,
  * uniform); labels: (M,) int32 packed MSB-first labels; 2 <= M <= 65536,
  * k = log2(M) <= 16.  out: (n, k) float32.
  * Replaces `llr` (commkit/_cuda/src/llr.cu:200-223; wrapper
- * commkit/_cuda/llr.py; call site commkit/mapping/_demap.py:305). */
+ * commkit/_cuda/llr.py:36; call site commkit/mapping/_demap.py:210). */
 int32_t ck_llr_points(const void* x, int64_t n, const void* points,
                       const double* log_prior, const int32_t* labels, int32_t M,
                       int32_t k, double inv_s2, int32_t exact, float* out,
