@@ -3,10 +3,10 @@
 ``points[i]`` is the point whose MSB-first bit label is the binary value of
 ``i``, at unit average power for QAM and PAM (PSK lies on the unit circle).
 Follows ``commkit/mapping/gray.py:31-60`` (``gray_code``), ``:99-156``
-(``_gray_points``), ``:226-298`` (PSK, ASK) and ``:301-338`` (square QAM:
+(``_gray_points``), ``:226-298`` (PSK, ASK), ``:301-338`` (square QAM:
 the high ``k/2`` bits select the in-phase level, the low ones the
-quadrature level).  Cross QAM (32, 128, ...) is outside the receive tail's
-formats and raises.
+quadrature level) and ``:341-425`` (cross QAM of an odd number of bits
+above 3: a quasi-Gray labelling, the same points for every label).
 """
 
 from __future__ import annotations
@@ -55,6 +55,48 @@ def _gray_qam_square(order: int) -> np.ndarray:
     return pam[s >> n] + 1j * pam[s & ((1 << n) - 1)]
 
 
+def _gray_rank(word: np.ndarray, bits: int) -> np.ndarray:
+    """Position of each Gray code word in the Gray sequence of ``bits`` bits
+    (the inverse of :func:`gray_code`): the prefix XOR of the word's bits
+    from the most significant down."""
+    rank = np.asarray(word, dtype=np.int64).copy()
+    shift = 1
+    while shift < bits:
+        rank ^= rank >> shift
+        shift <<= 1
+    return rank
+
+
+def _gray_qam_cross(order: int) -> np.ndarray:
+    """Cross QAM of ``k = log2(order)`` bits, k odd and at least 5.
+
+    A label's high ``ceil(k/2)`` bits place it in a column and its low
+    ``floor(k/2)`` bits in a row of a Gray-labelled rectangle of
+    ``2^ceil(k/2)`` columns by ``2^floor(k/2)`` rows (each index Gray
+    coded on its own).  The outer eighth of the columns on either side then
+    moves, turned a quarter, onto the top (left band) and bottom (right
+    band) of the central square of columns, which makes the cross: a point
+    of the left band at (column c, row r) goes to column r + (W - H) / 2,
+    row H + b - 1 - c; of the right band to the same column and row
+    W - b - 1 - c, where W, H are the rectangle's sides and b the band's
+    width.  Neighbours across a fold differ in more than one bit
+    (quasi-Gray).  Points are odd integers, centred: ``2 x - (W - 1)``.
+    """
+    k = int(np.log2(order))
+    col_bits, row_bits = (k + 1) // 2, k // 2
+    W, H = 1 << col_bits, 1 << row_bits
+    label = np.arange(order, dtype=np.int64)
+    col = _gray_rank(label >> row_bits, col_bits)
+    row = _gray_rank(label & (H - 1), row_bits)
+    band = W >> 3
+    left = col < band
+    right = col >= W - band
+    moved = left | right
+    new_col = np.where(moved, row + (W - H) // 2, col)
+    new_row = np.where(left, H + band - 1 - col, np.where(right, W - band - 1 - col, row))
+    return (2.0 * new_col - (W - 1)) + 1j * (2.0 * new_row - (H - 1))
+
+
 def _unit_power(points: np.ndarray) -> np.ndarray:
     rms = np.sqrt(np.mean(np.abs(points) ** 2))
     return points / rms
@@ -78,9 +120,7 @@ def gray_points(family: str, order: int, unipolar: bool = False) -> np.ndarray:
         elif k % 2 == 0:
             pts = _gray_qam_square(order)
         else:
-            raise ValueError(
-                f"cross QAM ({order}-QAM) is not among the receive tail's formats."
-            )
+            pts = _gray_qam_cross(order)
         pts = _unit_power(pts)
     else:
         raise ValueError(f"Unsupported modulation type: {family}")

@@ -38,6 +38,9 @@ CASES = {
     "256qam-shaped": (lambda: Constellation.qam(256).shaped(nu=0.05), 10 ** (-1.8)),
     "8psk": (lambda: Constellation.psk(8), 0.05),
     "8qam": (lambda: Constellation.qam(8), 0.05),
+    "32qam-cross": (lambda: Constellation.qam(32), 10 ** (-1.2)),
+    "128qam-cross": (lambda: Constellation.qam(128), 10 ** (-1.7)),
+    "1024qam": (lambda: Constellation.qam(1024), 10 ** (-2.8)),
 }
 
 
@@ -289,7 +292,8 @@ def test_llr_sweep_against_fp64(gpu, name, method):
     make, dbs = GRAY_SWEEP[name]
     c = make()
     if method == "maxlog" and c.pmf is None and name != "16qam-natural":
-        pytest.skip("uniform Gray max-log takes the closed form (tested above)")
+        pytest.skip("uniform Gray max-log takes the closed form, which "
+                    "test_llr_against_fp64 holds to FP64 for QPSK to 1024-QAM")  # fmt: skip
     for db in dbs:
         nv = 10 ** (-db / 10)
         x = _noisy(c, 4096 if c.order >= 1024 else 8192, nv, seed=int(db * 10) + 7)

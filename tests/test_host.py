@@ -216,6 +216,39 @@ class TestConstellation:
         np.testing.assert_array_equal(c._axes[0].bits, np.arange(k // 2))
         np.testing.assert_array_equal(c._axes[1].bits, np.arange(k // 2, k))
 
+    @pytest.mark.parametrize("order", [32, 128])
+    def test_cross_qam_matches_the_reference(self, order):
+        """Cross QAM: the point of every label equals the reference's
+        (golden vectors written from commkit's own _gray_qam_cross by
+        tests/golden/make_cross_qam.py), scaled to unit power; the points
+        form the cross (the 2^ceil(k/2) x 2^floor(k/2) rectangle with an
+        eighth of the columns folded on each side), and most nearest
+        neighbours differ in one bit (quasi-Gray)."""
+        import json
+        import os
+
+        with open(os.path.join(os.path.dirname(__file__), "golden", "cross_qam.json")) as f:
+            golden = np.array([complex(a, b) for a, b in json.load(f)[str(order)]])
+        c = Constellation.qam(order)
+        assert abs(c.power() - 1.0) < 1e-12
+        np.testing.assert_allclose(c.points, golden / np.sqrt(np.mean(np.abs(golden) ** 2)),
+                                   rtol=0, atol=1e-15)  # fmt: skip
+        # the cross: distinct odd-integer points within +-(W - 2b - 1) on both
+        # axes, none in a corner where both |I| and |Q| exceed H
+        k = int(np.log2(order))
+        W, H = 1 << ((k + 1) // 2), 1 << (k // 2)
+        side = W - 2 * (W // 8) - 1
+        grid = {(int(p.real), int(p.imag)) for p in golden}
+        assert len(grid) == order and all(i % 2 and q % 2 for i, q in grid)
+        assert max(max(abs(i), abs(q)) for i, q in grid) == side
+        assert not any(abs(i) > H and abs(q) > H for i, q in grid)
+        assert c._axes is None  # not separable: the M-point demapper
+        labels = c.bit_labels.astype(int)
+        d = np.abs(c.points[:, None] - c.points[None, :])
+        near = np.isclose(d, d[d > 0].min())
+        diff = (labels[:, None, :] != labels[None, :, :]).sum(axis=2)
+        assert (diff[near] == 1).mean() > 0.8
+
     def test_shaped_maxwell_boltzmann(self):
         c = Constellation.qam(256).shaped(nu=0.05)
         assert abs(c.pmf.sum() - 1) < 1e-12 and abs(c.power() - 1) < 1e-12
