@@ -402,10 +402,28 @@ def _check_schedule(schedule: str) -> None:
 
 
 def _warn_if_4_cycles(code: LDPCCode) -> None:
-    h = code.parity_check().astype(np.int32)
-    overlap = (h @ h.T).tocoo()
-    if np.any((overlap.data > 1) & (overlap.row != overlap.col)):
-        logger.warning("%r has 4-cycles; belief propagation degrades on them.", code)
+    """Log a warning when two bits share two checks (a cycle of length 4 in
+    the Tanner graph, on which belief propagation degrades).
+
+    Every bit contributes each pair of its checks; a pair seen at two
+    different bits closes a 4-cycle.
+    """
+    order = np.lexsort((code.checks, code.variables))  # by bit, then check
+    bits = code.variables[order].astype(np.int64)
+    chks = code.checks[order].astype(np.int64)
+    start = np.searchsorted(bits, np.arange(code.num_variables + 1))
+    deg = np.diff(start)
+    m = np.int64(code.num_checks)
+    seen: list[np.ndarray] = []
+    for d in np.unique(deg[deg >= 2]):
+        # the bits of degree d as rows of their d checks, all pairs (i < j)
+        rows = chks[start[:-1][deg == d][:, None] + np.arange(d)]
+        i, j = np.triu_indices(int(d), 1)
+        seen.append((rows[:, i] * m + rows[:, j]).ravel())
+    if seen:
+        pairs = np.concatenate(seen)
+        if np.unique(pairs).size < pairs.size:
+            logger.warning("%r has 4-cycles; belief propagation degrades on them.", code)
 
 
 def _first_fit(ptr: np.ndarray, variables: np.ndarray, n: int) -> np.ndarray:
@@ -430,40 +448,61 @@ def _first_fit(ptr: np.ndarray, variables: np.ndarray, n: int) -> np.ndarray:
     return color
 
 
+def _is_accumulator(m: int, p_rows: np.ndarray, p_cols: np.ndarray) -> bool:
+    """Whether the parity part (entries p_rows x p_cols of an m x m block)
+    is the accumulator: ones on the diagonal and just below it, nothing
+    else (parity bit i in checks i and i + 1, the last one in check m - 1)."""
+    if p_rows.size != 2 * m - 1:
+        return False
+    below = p_rows.astype(np.int64) - p_cols
+    diag = below == 0
+    return bool(np.all(diag | (below == 1)) and np.count_nonzero(diag) == m and
+                np.unique(p_rows.astype(np.int64) * m + p_cols).size == p_rows.size)
+
+
 def _build_encoder(code: LDPCCode) -> _Encoder:
+    """The systematic encoder of a code: the first of three forms that
+    applies (``commkit/coding/ldpc.py:445-479`` tries the same ones in the
+    same order).
+
+    - ``accumulate``: the last m bits are the parity and their part of H is
+      the accumulator; information first, parity by a running XOR of the
+      information syndrome.
+    - ``inverse``: the parity part is invertible over GF(2) (m up to
+      8192); information first, parity = Hp^-1 H_info u.
+    - ``dense``: otherwise, from H row-reduced with pivots taken from the
+      right: the pivot columns carry parity, the free columns the
+      information, and the reduced rows give each parity bit from them.
+    """
     m, n = code.num_checks, code.num_variables
-    rows, cols = code.checks, code.variables
     k = n - m
-    in_parity = cols >= k
-    p_rows, p_cols = rows[in_parity], cols[in_parity] - k
-    info_rows, info_cols = rows[~in_parity], cols[~in_parity]
-    expected = np.sort(
-        np.concatenate([np.arange(m) * (m + 1), np.arange(m - 1) * (m + 1) + m])
-    )
-    if p_rows.size == 2 * m - 1 and np.array_equal(
-        np.sort(p_rows.astype(np.int64) * m + p_cols), expected
-    ):
-        return _Encoder("accumulate", np.arange(k), np.arange(k, n), (info_rows, info_cols))
+    parity_entry = code.variables >= k
+    info_rows, info_cols = code.checks[~parity_entry], code.variables[~parity_entry]
+    p_rows, p_cols = code.checks[parity_entry], code.variables[parity_entry] - k
+    info, parity = np.arange(k), np.arange(k, n)
+    if _is_accumulator(m, p_rows, p_cols):
+        return _Encoder("accumulate", info, parity, (info_rows, info_cols))
+    inv = None
     if m <= _MAX_INVERSE:
         hp = np.zeros((m, m), dtype=np.int8)
         hp[p_rows, p_cols] = 1
         try:
             inv = inverse(hp)
-        except ValueError:
-            pass
-        else:
-            return _Encoder("inverse", np.arange(k), np.arange(k, n), (info_rows, info_cols, inv))
+        except ValueError:  # singular: the dense form below
+            inv = None
+    if inv is not None:
+        return _Encoder("inverse", info, parity, (info_rows, info_cols, inv))
     if m * n > _MAX_DENSE:
         raise ValueError(
             f"LDPCCode ({m} x {n}): the parity part is neither dual-diagonal nor "
             "invertible, and H is too large for a dense systematic generator."
         )
     h = np.zeros((m, n), dtype=np.int8)
-    h[rows, cols] = 1
-    rref, pivots = row_reduce(h, reverse=True)
+    h[code.checks, code.variables] = 1
+    reduced, pivots = row_reduce(h, reverse=True)
     free = np.setdiff1d(np.arange(n), pivots)
-    # c[pivots[i]] = sum over free f of rref[i, f] c[f]
-    return _Encoder("dense", free, pivots, (rref[:, free].T.astype(np.int8),))
+    # row i of the reduction: c[pivots[i]] = XOR over free f of reduced[i, f] c[f]
+    return _Encoder("dense", free, pivots, (reduced[:, free].T.astype(np.int8),))
 
 
 # -----------------------------------------------------------------------------

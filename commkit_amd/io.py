@@ -98,7 +98,7 @@ def save_npz(signal: Signal, path: str | Path, *, compressed: bool = True) -> No
             if info is not None:
                 arrays["reference_info_bits"] = _host(info)
             arrays["reference_code_block"] = np.array([ref.code.n, ref.code.k], dtype=np.int64)
-            meta["reference_code"] = _put_code(ref.code, "reference_code", arrays)
+            meta["reference_code"] = _code_to_form(ref.code, "reference_code", arrays)
     _put_constellation(signal.constellation, "constellation", arrays, meta)
     arrays["__metadata__"] = _json_array(meta)
     (np.savez_compressed if compressed else np.savez)(path, **arrays)
@@ -146,7 +146,7 @@ def _read_signal(data: Any, path: Path) -> Signal:
             symbols=data["reference_symbols"] if "reference_symbols" in data else None,
             bits=bits,
             info_bits=info if info is not None else bits,
-            code=_get_code(meta.get("reference_code"), data, path),
+            code=_code_from_form(meta.get("reference_code"), data, path),
         )
     return Signal(
         samples=data["samples"],
@@ -200,47 +200,63 @@ def _code_types() -> dict[str, type]:
     return {cls.__name__: cls for cls in (LDPCCode, CCDM, PAS)}
 
 
-def _init_fields(obj: Any) -> dict:
-    return {f.name: getattr(obj, f.name) for f in dataclasses.fields(obj) if f.init}
+# A code in the archive is a tree of JSON forms, the layout the reference's
+# archives use, one form per kind of field (the key names the form):
+#   None, bool, int, float, str   as themselves (NumPy scalars as Python's);
+#   tuple                         {"tuple": [form, ...]};
+#   ndarray                       {"array": name}, the array stored as `name`;
+#   Constellation                 {"constellation": prefix, "family": family},
+#                                 its arrays stored as `prefix_points`, ...;
+#   LDPCCode, CCDM, PAS           {"type": class name, "fields": {name: form}},
+#                                 over the constructor's fields.
+# Array names extend the parent's name by the field path ("reference_code",
+# then "reference_code.checks", "reference_code.matcher.composition", ...).
 
 
-def _put_code(value: Any, key: str, arrays: dict) -> Any:
-    """``value`` (a code or one of its fields) as JSON; arrays go to
-    ``arrays`` under ``key`` extended by the field's dotted path."""
+def _code_to_form(value: Any, name: str, arrays: dict) -> Any:
+    """The JSON form of ``value`` (a code or a field of one); its arrays are
+    added to ``arrays``."""
     if value is None or isinstance(value, (bool, int, float, str)):
         return value
     if isinstance(value, np.generic):
         return value.item()
-    if isinstance(value, tuple):
-        return {"tuple": [_put_code(v, f"{key}.{i}", arrays) for i, v in enumerate(value)]}
     if isinstance(value, np.ndarray):
-        arrays[key] = value
-        return {"array": key}
+        arrays[name] = value
+        return {"array": name}
+    if isinstance(value, tuple):
+        return {"tuple": [_code_to_form(item, f"{name}.{i}", arrays)
+                          for i, item in enumerate(value)]}  # fmt: skip
     if isinstance(value, Constellation):
         meta: dict = {}
-        _put_constellation(value, key, arrays, meta)
-        return {"constellation": key, "family": meta[f"{key}_family"]}
-    name = type(value).__name__
-    if _code_types().get(name) is type(value):
-        fields = {f: _put_code(v, f"{key}.{f}", arrays) for f, v in _init_fields(value).items()}
-        return {"type": name, "fields": fields}
-    raise ValueError(f"save_npz(): cannot store a {name} in a code.")
+        _put_constellation(value, name, arrays, meta)
+        return {"constellation": name, "family": meta[f"{name}_family"]}
+    cls = type(value)
+    if _code_types().get(cls.__name__) is not cls:
+        raise ValueError(f"save_npz(): cannot store a {cls.__name__} in a code.")
+    fields = {}
+    for f in dataclasses.fields(value):
+        if f.init:
+            fields[f.name] = _code_to_form(getattr(value, f.name), f"{name}.{f.name}", arrays)
+    return {"type": cls.__name__, "fields": fields}
 
 
-def _get_code(value: Any, data: Any, path: Path) -> Any:
-    if not isinstance(value, dict):
-        return value
-    if "tuple" in value:
-        return tuple(_get_code(v, data, path) for v in value["tuple"])
-    if "array" in value:
-        return data[value["array"]]
-    if "constellation" in value:
-        key = value["constellation"]
-        return _get_constellation(key, data, {f"{key}_family": value["family"]})
-    cls = _code_types().get(value.get("type"))
+def _code_from_form(form: Any, data: Any, path: Path) -> Any:
+    """The value a JSON form of :func:`_code_to_form` stands for."""
+    if not isinstance(form, dict):
+        return form
+    if "array" in form:
+        return data[form["array"]]
+    if "tuple" in form:
+        return tuple(_code_from_form(item, data, path) for item in form["tuple"])
+    if "constellation" in form:
+        prefix = form["constellation"]
+        return _get_constellation(prefix, data, {f"{prefix}_family": form["family"]})
+    cls = _code_types().get(form.get("type"))
     if cls is None:
-        raise ValueError(f"Unknown code type {value.get('type')!r} in {path}.")
-    code = cls(**{f: _get_code(v, data, path) for f, v in value["fields"].items()})
+        # the reference's message (part of the load_npz contract)
+        raise ValueError(f"Unknown code type {form.get('type')!r} in {path}.")
+    kwargs = {k: _code_from_form(v, data, path) for k, v in form["fields"].items()}
+    code = cls(**kwargs)
     return _quasi_cyclic(code) if cls.__name__ == "LDPCCode" else code
 
 

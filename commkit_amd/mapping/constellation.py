@@ -21,7 +21,7 @@ import numpy as np
 from .._array import require_finite, require_int, warn_if_double
 from ..backend import device_tables, dispatch, to_host_like
 from .gray import gray_points
-from .lattice import Axis, find_separable_axes, find_square_grid
+from .lattice import Axis, separable_axes
 
 __all__ = ["Constellation", "pack_bits", "unpack_bits"]
 
@@ -255,12 +255,8 @@ class Constellation:
     # -- cached structure ------------------------------------------------------------
 
     @cached_property
-    def _square_grid(self) -> tuple[float, float, int, np.ndarray] | None:
-        return find_square_grid(self.points)
-
-    @cached_property
     def _axes(self) -> tuple[Axis, Axis] | None:
-        return find_separable_axes(self.points, self.bit_labels, self.pmf)
+        return separable_axes(self.points, self.bit_labels, self.pmf)
 
     @cached_property
     def _points_by_label(self) -> np.ndarray:

@@ -1,11 +1,19 @@
-"""Square-lattice structure of a constellation (host).
+"""Per-axis factoring of a constellation (host).
 
-``find_square_grid`` tells whether the points form a full square lattice
-with the same levels on both axes, and ``find_separable_axes`` whether the
-labels and the prior of that lattice factor into per-axis ones (Gray QAM,
-shaped or not).  Every bit of a separable constellation depends on one axis,
-so its LLR needs only that axis's levels: the demapper's fast paths.
-Follows ``commkit/mapping/_lattice.py:24-31,73-124``.
+A constellation is *separable* when its points are a full ``S x S`` square
+lattice (the same ``S`` equally spaced levels on the real and the imaginary
+axis, every combination present once), every label bit is a function of one
+axis's level only, each axis carries ``log2 S`` bits, and the prior (if any)
+is the product of a prior per axis.  Then a bit's LLR depends on one axis
+alone, which is what the demapper's per-axis kernels use: Gray square QAM,
+uniform or Maxwell-Boltzmann shaped, is separable; PSK, cross QAM and
+8-QAM are not.
+
+The factoring is the one of ``commkit/mapping/_lattice.py:24-31,73-124``
+(same ``Axis`` fields, levels ascending and taken from the points
+themselves, marginal priors summed over the other axis, a relative 1e-9
+tolerance on lattice positions and on the product prior), reached here by
+indexing every point with its lattice coordinates.
 """
 
 from __future__ import annotations
@@ -14,7 +22,11 @@ from typing import NamedTuple
 
 import numpy as np
 
-__all__ = ["Axis", "find_square_grid", "find_separable_axes"]
+__all__ = ["Axis", "lattice_coordinates", "separable_axes"]
+
+# Largest distance from a lattice position, in units of the level spacing,
+# and relative tolerance of the product-prior check.
+_TOL = 1e-9
 
 
 class Axis(NamedTuple):
@@ -26,58 +38,66 @@ class Axis(NamedTuple):
     bits: np.ndarray  # (k_axis,) positions of these bits in the symbol label
 
 
-def find_square_grid(points: np.ndarray) -> tuple[float, float, int, np.ndarray] | None:
-    """``(lev_min, step, side, lut)`` if ``points`` is a full square lattice.
+def _snap(values: np.ndarray, origin: float, spacing: float, side: int) -> np.ndarray | None:
+    """Lattice index of each value (origin + index * spacing), or None when a
+    value is off the lattice or outside its ``side`` positions."""
+    pos = (values - origin) / spacing
+    index = np.rint(pos)
+    if np.abs(pos - index).max() > _TOL or index.min() < 0 or index.max() > side - 1:
+        return None
+    return index.astype(np.int64)
 
-    ``lut[i * side + q]`` is the index of the point at level ``i`` on the
-    real axis and ``q`` on the imaginary one.
-    """
-    order = points.size
-    side = int(round(order**0.5))
-    if not np.iscomplexobj(points) or side * side != order or side < 2:
+
+def lattice_coordinates(points: np.ndarray) -> tuple[int, np.ndarray] | None:
+    """``(S, where)`` when ``points`` form a full ``S x S`` square lattice:
+    ``where[i, q]`` is the index of the point on real level ``i`` and
+    imaginary level ``q`` (levels counted from the lowest); else None."""
+    points = np.asarray(points)
+    side = int(np.rint(np.sqrt(points.size)))
+    if side < 2 or side * side != points.size or not np.iscomplexobj(points):
         return None
     re, im = points.real, points.imag
-    lev_min = float(re.min())
-    step = (float(re.max()) - lev_min) / (side - 1)
-    if step <= 0 or abs(float(im.min()) - lev_min) > 1e-9 * step:
+    origin = float(re.min())
+    spacing = (float(re.max()) - origin) / (side - 1)
+    if not spacing > 0.0:
         return None
-    gi, gq = (re - lev_min) / step, (im - lev_min) / step
-    ri, rq = np.round(gi), np.round(gq)
-    if max(np.abs(gi - ri).max(), np.abs(gq - rq).max()) > 1e-9:
+    col = _snap(re, origin, spacing, side)
+    row = _snap(im, origin, spacing, side)
+    if col is None or row is None:
         return None
-    cell = ri.astype(np.int64) * side + rq.astype(np.int64)
-    if cell.min() < 0 or cell.max() >= order or np.unique(cell).size != order:
+    where = np.full((side, side), -1, dtype=np.int64)
+    where[col, row] = np.arange(points.size)
+    if (where < 0).any():  # a lattice position taken twice, another left empty
         return None
-    lut = np.empty(order, dtype=np.int64)
-    lut[cell] = np.arange(order)
-    lut.setflags(write=False)
-    return lev_min, step, side, lut
+    return side, where
 
 
-def find_separable_axes(
-    points: np.ndarray, bit_labels: np.ndarray, pmf: np.ndarray | None
-) -> tuple[Axis, Axis] | None:
-    """Per-axis factors of a separable square lattice, or ``None``."""
-    grid = find_square_grid(points)
+def separable_axes(points: np.ndarray, bit_labels: np.ndarray,
+                   pmf: np.ndarray | None) -> tuple[Axis, Axis] | None:
+    """The two axes of a separable constellation, or None."""
+    grid = lattice_coordinates(points)
     if grid is None:
         return None
-    _, _, side, lut = grid
-    labels = bit_labels[lut].reshape(side, side, -1)  # [i, q, bit]
-    on_re = np.all(labels == labels[:, :1, :], axis=(0, 1))  # constant in q
-    on_im = np.all(labels == labels[:1, :, :], axis=(0, 1))  # constant in i
-    if not np.all(on_re ^ on_im):
+    side, where = grid
+    table = np.asarray(bit_labels)[where]  # (S, S, k): labels by (real, imaginary) level
+    # A real-axis bit is the same along every row of the table, an
+    # imaginary-axis bit the same along every column; each bit is exactly one.
+    along_im = (table == table[:, :1, :]).all(axis=(0, 1))
+    along_re = (table == table[:1, :, :]).all(axis=(0, 1))
+    if not np.all(along_im != along_re):
         return None
-    bits_re, bits_im = np.flatnonzero(on_re), np.flatnonzero(on_im)
-    if 2**bits_re.size != side or 2**bits_im.size != side:
+    re_bits = np.flatnonzero(along_im)
+    im_bits = np.flatnonzero(along_re)
+    if (1 << re_bits.size) != side or (1 << im_bits.size) != side:
         return None
-    grid_points = points[lut].reshape(side, side)
-    pmf_re = pmf_im = None
+    re_pmf = im_pmf = None
     if pmf is not None:
-        p = pmf[lut].reshape(side, side)
-        pmf_re, pmf_im = p.sum(axis=1), p.sum(axis=0)
-        if not np.allclose(p, np.outer(pmf_re, pmf_im), rtol=1e-9, atol=0.0):
+        joint = np.asarray(pmf)[where]
+        re_pmf, im_pmf = joint.sum(axis=1), joint.sum(axis=0)
+        if not np.allclose(joint, np.outer(re_pmf, im_pmf), rtol=_TOL, atol=0.0):
             return None
+    pts = np.asarray(points)
     return (
-        Axis(grid_points[:, 0].real.copy(), labels[:, 0][:, bits_re], pmf_re, bits_re),
-        Axis(grid_points[0, :].imag.copy(), labels[0, :][:, bits_im], pmf_im, bits_im),
+        Axis(pts[where[:, 0]].real.copy(), table[:, 0][:, re_bits], re_pmf, re_bits),
+        Axis(pts[where[0, :]].imag.copy(), table[0, :][:, im_bits], im_pmf, im_bits),
     )
