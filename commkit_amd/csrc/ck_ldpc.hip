@@ -51,6 +51,7 @@
 #include "ck_ldpc_dev.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace {
 
@@ -876,4 +877,76 @@ extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method
         }
     }
     return ck::launched("ldpc generic decode");
+}
+
+// ---------------------------------------------------------------------------
+// Decoding without allocation: the workspace is a fixed device array
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Bytes of the static scratch: 64 MiB, held by every device the library runs
+// on until the process ends.
+constexpr size_t kScratchBytes = static_cast<size_t>(64) << 20;
+constexpr int kScratchDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch[kScratchDevices] = {};
+
+// The device address of `scratch` on the current device, resolved once per
+// device (the array's name is a host-side stand-in, not a device address).
+int32_t scratch_address(void** out) {
+    int dev = 0;
+    CK_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kScratchDevices)
+        return ck::fail(CK_ENOSPACE, "device index %d beyond %d", dev, kScratchDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[dev] == nullptr) {
+        void* p = nullptr;
+        CK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (p == nullptr) return ck::fail(CK_EHIP, "hipGetSymbolAddress(scratch) gave NULL");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return CK_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ck_ldpc_decode_static(const ck_ldpc_code* code, const ck_ldpc_method* method,
+                                         const float* llrs, float* app, int8_t* bits,
+                                         int32_t* iterations, uint8_t* valid, int64_t R,
+                                         void* stream) {
+    CK_REQUIRE(R >= 0, "R = %lld < 0", static_cast<long long>(R));
+    Plan p{};
+    if (int32_t st = make_plan(code, method, R, &p)) return st;
+    if (R == 0) return CK_OK;
+    if (p.ws_per_frame == 0)  // the check state lives in LDS: no workspace at all
+        return ck_ldpc_decode(code, method, llrs, app, bits, iterations, valid, R, nullptr, stream);
+    CK_REQUIRE(p.ws_per_frame <= kScratchBytes,
+               "one frame needs %zu bytes of workspace; the static scratch holds %zu",
+               p.ws_per_frame, kScratchBytes);
+    CK_REQUIRE(llrs && app && iterations && valid, "NULL input or output");
+    void* ws = nullptr;
+    if (int32_t st = scratch_address(&ws)) return st;
+    // Frames in chunks that fit the scratch, one launch after the other on
+    // `stream`: a chunk's plan never needs more workspace per frame than the
+    // whole batch's (the check state leaves LDS only for more frames per
+    // workgroup), which the check below confirms before each launch.
+    const long long chunk = static_cast<long long>(kScratchBytes / p.ws_per_frame);
+    const ck_ldpc_code& c = *code;
+    for (long long r0 = 0; r0 < R; r0 += chunk) {
+        const long long rc = std::min<long long>(chunk, R - r0);
+        const int64_t need = ck_ldpc_workspace_bytes(code, method, rc);
+        CK_REQUIRE(need >= 0 && static_cast<size_t>(need) <= kScratchBytes,
+                   "a chunk of %lld frames needs %lld bytes of workspace; the static scratch "
+                   "holds %zu", rc, static_cast<long long>(need), kScratchBytes);
+        if (int32_t st = ck_ldpc_decode(code, method, llrs + r0 * c.n, app + r0 * c.n,
+                                        bits ? bits + r0 * c.k : nullptr, iterations + r0,
+                                        valid + r0, rc, need > 0 ? ws : nullptr, stream))
+            return st;
+    }
+    return CK_OK;
 }

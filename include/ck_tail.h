@@ -205,6 +205,21 @@ int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method* method,
                        int32_t* iterations, uint8_t* valid, int64_t R,
                        void* workspace, void* stream);
 
+/* ck_ldpc_decode without a workspace argument and without allocating: the
+ * workspace is a static device array of 64 MiB (each GPU the library runs
+ * on holds one until the process ends), and frames are decoded in chunks
+ * that fit it, one launch after the other on `stream`; a frame needing more
+ * than 64 MiB of workspace is refused (CK_EINVAL).  Nothing in the scratch
+ * must start at zero (the first iteration reads no old message).  Calls
+ * that overlap on one GPU (other streams, other threads) share the scratch
+ * and race: serialise them.  Outputs, validation and results are those of
+ * ck_ldpc_decode.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+int32_t ck_ldpc_decode_static(const ck_ldpc_code* code, const ck_ldpc_method* method,
+                              const float* llrs, float* app, int8_t* bits,
+                              int32_t* iterations, uint8_t* valid, int64_t R, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched decoding: frames of several QC codes in one persistent launch
  * ------------------------------------------------------------------------ */
