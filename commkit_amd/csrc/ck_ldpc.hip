@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
             for (int v = l.u; v < N; v += Z * P) {
                 const int j = v / Z;
                 const int cc = v - j * Z;
-                float sum = v < c.n ? L[v] : 0.0f;
+                float sum = v < c.n ? L[v] + 0.0f : 0.0f;  // -0.0 as +0.0, as at the load
                 for (int t = c.qc_col_ptr[j]; t < c.qc_col_ptr[j + 1]; ++t) {
                     const int s = c.qc_col_slot[t];
                     int rr = cc - c.qc_slot_shift[s];
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kGenericThreads) void ldpc_generic(
                 generic_row<SP, false, MAXD, false>(c, A, Rm, row, first, prm);
             __syncthreads();
             for (int v = threadIdx.x; v < N; v += blockDim.x) {
-                float sum = v < c.n ? L[v] : 0.0f;
+                float sum = v < c.n ? L[v] + 0.0f : 0.0f;  // -0.0 as +0.0, as at the load
                 for (int t = c.var_ptr[v]; t < c.var_ptr[v + 1]; ++t) sum = sum + Rm[c.var_edges[t]];
                 A.st(v, sum);
             }

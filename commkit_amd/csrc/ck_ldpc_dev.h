@@ -67,6 +67,13 @@ __device__ __forceinline__ float state_msg(const CheckState& st, int e) {
     return __uint_as_float(__float_as_uint(mag) | ((st.signs << (31 - e)) & 0x80000000u));
 }
 
+// x + 0.0f per component: -0.0 becomes +0.0, every other value stays (the
+// decoders read a Q's sign from its sign bit, and no Q is -0.0 once no
+// channel LLR is; the reference's comparisons treat -0.0 and +0.0 alike).
+__device__ __forceinline__ float4 plus_zero(float4 v) {
+    return make_float4(v.x + 0.0f, v.y + 0.0f, v.z + 0.0f, v.w + 0.0f);
+}
+
 // Channel LLRs of frames [frame0, frame0 + F) into LDS (zeros for the
 // punctured tail and for frames past R).  The 16-byte path walks the F
 // frames as one range of F * N/4 vectors and issues up to kLoadBatch of a
@@ -98,6 +105,7 @@ __device__ inline void load_frames(float* lds, const float* __restrict__ llr, lo
                 const bool load = q < total && j < m4 && frame < R;
                 v[u] = load ? reinterpret_cast<const float4*>(llr + frame * n)[j]
                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                v[u] = plus_zero(v[u]);
                 q += step;
                 j += step;
                 while (j >= n4) {
@@ -119,7 +127,7 @@ __device__ inline void load_frames(float* lds, const float* __restrict__ llr, lo
             continue;
         }
         const float* src = llr + frame * n;
-        for (int v = threadIdx.x; v < N; v += blockDim.x) dst[v] = v < n ? src[v] : 0.0f;
+        for (int v = threadIdx.x; v < N; v += blockDim.x) dst[v] = v < n ? src[v] + 0.0f : 0.0f;
     }
 }
 
@@ -139,7 +147,7 @@ __device__ inline void load_frame(float* a, const float* __restrict__ src, int n
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 const int j = j0 + k * lanes;
-                v[k] = j < m4 ? s4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                v[k] = plus_zero(j < m4 ? s4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
@@ -154,7 +162,7 @@ __device__ inline void load_frame(float* a, const float* __restrict__ src, int n
 #pragma unroll
         for (int k = 0; k < kLoadBatch; ++k) {
             const int v = v0 + k * lanes;
-            x[k] = v < n ? src[v] : 0.0f;
+            x[k] = v < n ? src[v] + 0.0f : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < kLoadBatch; ++k) {
@@ -497,10 +505,29 @@ constexpr int kSlotStep = 2;
 // the frame).  UPDATE_APP: layered (APP = Q + R_new); else the flooding check
 // phase, which leaves the APP alone.  Q and the LDS address of every edge
 // stay in registers between the two passes.
+//
+// The rule is the sequential one of the reference (min1 = the smallest |Q|,
+// min2 the second smallest, R_e = sign * (e == argmin ? m2 : m1) with
+// sign = parity of the row's Q signs ^ (Q_e < 0)), evaluated so that an edge
+// costs few instructions:
+//   - pass 1 keeps only min1 and min2 (min and med3 return one of their
+//     operands, so nothing is rounded); the argmin is found in pass 2 as an
+//     edge whose |Q| equals min1.  With one smallest |Q| that edge is the
+//     reference's argmin; with a tie min2 = min1, so m2 = m1 and every edge
+//     gets the same message whichever tied edge is called the argmin;
+//   - Q_e < 0 is Q_e's sign bit: no Q is -0.0, because the channel LLRs are
+//     loaded with -0.0 turned into +0.0 (x + 0.0f) and a - b or a + b is
+//     -0.0 only when a is -0.0 (round to nearest), so no APP and no Q ever
+//     becomes one.  With one lane per row (P = 1) the sign bits are shifted
+//     into a word one alignbit per edge (edge t lands at bit NT - 1 - t) and
+//     bit-reversed into edge order once per row;
+//   - R_e's sign bit is the row parity's flipped by Q_e's sign bit: the
+//     magnitude with the parity's sign bit, XORed with Q_e & 0x80000000.
 template <int P, int NT, bool UPDATE_APP>
 __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, const CheckState st,
                                                   int s0, int dc, int r, int p, int Z,
                                                   const Params& prm) {
+    constexpr int S = kSlotStep<P>;
     float q[NT];
     float* addr[NT];
     float* ar = a + r;
@@ -511,77 +538,80 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
         addr[t] = &lds_at(ar, qc_off(tab[s0 + t * P + p], rr, z4));
         q[t] = *addr[t];
     }
-    // Pass 1.  min1 / min2 / idx follow the sequential rule exactly:
-    //   idx moves on a strictly smaller |Q| (the first edge attaining min1),
-    //   min2 = min(min2, max(min1, |Q|)) = med3(min1, min2, |Q|) as min1 <= min2,
-    //   min1 = min(min1, |Q|)
-    // (min, max and med3 return one of their operands, so nothing is rounded).
-    // Bit e of qneg is Q_e < 0; the sign parity is its popcount.
+    // Pass 1: Q, min1 and min2 (min2 = med3(min1, min2, |Q|) as min1 <= min2),
+    // the sign bits of Q.
     float min1 = ck::kInf;
     float min2 = ck::kInf;
-    int idx = 0x7fffffff;
-    unsigned qneg = 0u;
+    unsigned qneg = 0u;  // P > 1: bit e is Q_e < 0
+    unsigned qrev = 0u;  // P = 1: sign bit of Q_t at bit NT - 1 - t
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
-        const bool valid = (t < NT - kSlotStep<P>) || (e < dc);
+        const bool valid = (t < NT - S) || (e < dc);
         const float qv = q[t] - state_msg(st, e);
         q[t] = qv;
         const float mag = valid ? fabsf(qv) : ck::kInf;
-        idx = (mag < min1) ? e : idx;
         min2 = __builtin_amdgcn_fmed3f(min1, min2, mag);
         min1 = fminf(min1, mag);
-        qneg |= (valid && qv < 0.0f) ? (1u << e) : 0u;
+        if constexpr (P == 1)
+            qrev = __builtin_amdgcn_alignbit(qrev, __float_as_uint(qv), 31u);
+        else
+            qneg |= (valid && qv < 0.0f) ? (1u << e) : 0u;
     }
-    // Lanes of a row merge their (min1, min2, idx) by the same rule, which
-    // gives the same result whichever side is taken as the lane's own.
-    if constexpr (P == 2) {
+    const unsigned row_mask = (dc >= 32) ? 0xffffffffu : ((1u << dc) - 1u);
+    if constexpr (P == 1) {
+        qneg = (__builtin_bitreverse32(qrev) >> (32 - NT)) & row_mask;
+    } else if constexpr (P == 2) {
+        // The row's two smallest |Q| from each lane's two smallest.
         const int2 v1 = row_pair(__float_as_int(min1));
         const int2 v2 = row_pair(__float_as_int(min2));
-        const int2 vi = row_pair(idx);
         const int2 vq = row_pair(static_cast<int>(qneg));
         const float x1 = __int_as_float(v1.x);
         const float y1 = __int_as_float(v1.y);
-        if (y1 < x1 || (y1 == x1 && vi.y < vi.x)) {
-            min2 = fminf(x1, __int_as_float(v2.y));
-            min1 = y1;
-            idx = vi.y;
-        } else {
-            min2 = fminf(__int_as_float(v2.x), y1);
-            min1 = x1;
-            idx = vi.x;
-        }
+        min1 = fminf(x1, y1);
+        min2 = fminf(fmaxf(x1, y1), fminf(__int_as_float(v2.x), __int_as_float(v2.y)));
         qneg = static_cast<unsigned>(vq.x | vq.y);
     } else {
 #pragma unroll
         for (int off = 1; off < P; off <<= 1) {
             const float o1 = __int_as_float(lane_xor(__float_as_int(min1), off));
             const float o2 = __int_as_float(lane_xor(__float_as_int(min2), off));
-            const int oi = lane_xor(idx, off);
             qneg |= static_cast<unsigned>(lane_xor(static_cast<int>(qneg), off));
-            if (o1 < min1 || (o1 == min1 && oi < idx)) {
-                min2 = fminf(min1, o2);
-                min1 = o1;
-                idx = oi;
-            } else {
-                min2 = fminf(min2, o1);
-            }
+            min2 = fminf(fmaxf(min1, o1), fminf(min2, o2));
+            min1 = fminf(min1, o1);
         }
     }
     const float m1 = fmaxf(prm.scale * min1 - prm.offset, 0.0f);
     const float m2 = fmaxf(prm.scale * min2 - prm.offset, 0.0f);
-    // Pass 2: R_e is negative where (total parity) ^ (Q_e < 0).
-    const unsigned row_mask = (dc >= 32) ? 0xffffffffu : ((1u << dc) - 1u);
-    const unsigned signs = ((__popc(qneg) & 1) ? ~qneg : qneg) & row_mask;
-    if (UPDATE_APP) {
+    const unsigned parity = static_cast<unsigned>(__popc(qneg) & 1);
+    const unsigned signs = (parity ? ~qneg : qneg) & row_mask;
+    // Pass 2: the argmin (an edge at min1), and with UPDATE_APP the new APP.
+    const unsigned pbit = parity << 31;
+    float m1p = __uint_as_float(__float_as_uint(m1) | pbit);
+    float m2p = __uint_as_float(__float_as_uint(m2) | pbit);
+    // Opaque to the compiler, so that an edge's sign costs one bitop3
+    // (mp ^ (Q & 0x80000000)) instead of the parity folded in per edge.
+    asm volatile("" : "+v"(m1p), "+v"(m2p));
+    int idx = -1;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int e = t * P + p;
-            if ((t < NT - kSlotStep<P>) || (e < dc)) {
-                const float mag = (e == idx) ? m2 : m1;
-                *addr[t] = q[t] + with_sign(mag, signs, e);
-            }
+    for (int t = 0; t < NT; ++t) {
+        const int e = t * P + p;
+        const bool valid = (t < NT - S) || (e < dc);
+        const bool at_min = valid && fabsf(q[t]) == min1;
+        idx = at_min ? e : idx;
+        if (UPDATE_APP && valid) {
+            const float mp = at_min ? m2p : m1p;
+            const float rn = __uint_as_float(__float_as_uint(mp) ^
+                                             (__float_as_uint(q[t]) & 0x80000000u));
+            *addr[t] = q[t] + rn;
         }
+    }
+    if constexpr (P == 2) {
+        const int2 vi = row_pair(idx);
+        idx = max(vi.x, vi.y);
+    } else {
+#pragma unroll
+        for (int off = 1; off < P; off <<= 1) idx = max(idx, lane_xor(idx, off));
     }
     CheckState out;
     out.m1 = m1;

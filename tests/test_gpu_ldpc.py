@@ -243,3 +243,23 @@ def test_infeasible_tuned_tile_falls_back(gpu, monkeypatch):
     assert_identical(gpu_decode(code, llrs, method), oracle_decode(code, llrs, method), "oracle")
     with pytest.raises(RuntimeError, match="LDS"):
         gpu_decode(code, llrs, method, (2, 2, 1, 1))
+
+
+@pytest.mark.parametrize("schedule", ["layered", "flooding"])
+def test_signed_zero_llrs(gpu, schedule):
+    """Channel LLRs of exactly -0.0 and +0.0 (a symbol on a decision
+    boundary): the decoder takes -0.0 as +0.0, as every comparison of the
+    reference does, so bits, iterations and valid equal the oracle's and the
+    APP LLRs equal it as values (a zero APP may differ in its sign bit)."""
+    code = seeded_code("r2/3-z96")
+    llrs = _llrs(code, 6, 1.2, seed=12)
+    r = np.random.default_rng(13)
+    hit = r.random(llrs.shape) < 0.05
+    llrs[hit] = np.where(r.random(hit.sum()) < 0.5, -0.0, 0.0).astype(np.float32)
+    assert np.signbit(llrs[hit]).any() and (~np.signbit(llrs[hit])).any()
+    method = MinSum(scale=0.75, max_iterations=15, schedule=schedule)
+    want = oracle_decode(code, llrs, method)
+    for tiles in (None, (2, 1, 1, 0), (2, 1, 2, 0), (2, 1, 4, 0)):
+        got = gpu_decode(code, llrs, method, tiles)
+        for name, g, w in zip(("bits", "llrs", "valid", "iterations"), got, want):
+            assert np.array_equal(g, w), f"{schedule} {tiles}: {name}"
