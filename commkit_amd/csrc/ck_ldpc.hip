@@ -108,10 +108,13 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
     for (;;) {
         // The syndrome is checked before the first iteration and after each
         // one with early stop, else once after the last (one call site keeps
-        // the kernel's registers down).
+        // the kernel's registers down).  After an iteration of min-sum the
+        // last block row's hard-decision parities (final: no later row of
+        // the iteration touches its bits) have already settled s_flag.
         const bool last = it >= prm.max_iterations;
         if (prm.early_stop || last) {
-            const bool now = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, !done, F, s_flag);
+            const bool now = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, !done, F, s_flag,
+                                            !SP && it > 0 && mb > 1);
             if (!done) {
                 ok = now;
                 if (prm.early_stop) done = now;
@@ -119,6 +122,10 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
         }
         if (last || !__syncthreads_or(!done)) break;
         const bool first = it == 0;
+        // Cleared for the parities of this iteration's last block row (the
+        // barrier above ordered the syndrome's reads before this write, the
+        // barriers after the block rows order it before theirs).
+        if (!SP && mb > 1 && lane_ok && l.u == 0) s_flag[l.f] = 0;
         for (int br = 0; br < mb; ++br) {
             if (!done) {
                 if (SP) {
@@ -131,8 +138,16 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
                     } else if (mb > 1) {
                         st_next = st_base[l.r];  // block row 0, stored this iteration
                     }
-                    const CheckState ns =
-                        qc_ms_row<P, true, kRowSlots<P, SHORT>>(c, s_tab, a, st, br, l.r, l.p, prm);
+                    CheckState ns;
+                    if (br + 1 < mb || mb == 1) {
+                        ns = qc_ms_row<P, true, kRowSlots<P, SHORT>>(c, s_tab, a, st, br, l.r, l.p,
+                                                                     prm);
+                    } else {
+                        unsigned odd = 0u;
+                        ns = qc_ms_row<P, true, kRowSlots<P, SHORT>, true>(c, s_tab, a, st, br,
+                                                                           l.r, l.p, prm, &odd);
+                        if (odd) s_flag[l.f] = 1;
+                    }
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
                 }

@@ -337,6 +337,32 @@ __device__ __forceinline__ float with_sign(float mag, unsigned bits, int e) {
     return __uint_as_float(__float_as_uint(mag) | ((bits << (31 - e)) & 0x80000000u));
 }
 
+// Block rows checked between barriers by the direct syndrome check.
+constexpr int kSynGroup = 3;
+
+// Parity (0 or 1) of the hard decisions of row r of block row br of the
+// frame whose APP starts at `a`: the XOR of the APPs' sign bits over the
+// row's circulants (no APP is -0.0, so a sign bit is APP < 0).  Table
+// entries past the row's end are real slots or padding, in the frame.
+__device__ __forceinline__ unsigned qc_row_parity(const ck_ldpc_code& c, const int2* tab,
+                                                  const float* a, int br, int r) {
+    constexpr int U = 8;
+    const int s0 = ld_uniform(c.qc_row_ptr, br);
+    const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
+    float* ar = const_cast<float*>(a) + r;
+    const unsigned rr = row_rank(r);
+    const int z4 = neg_col_bytes(c.lifting);
+    unsigned acc = 0u;
+    for (int t = 0; t < dc; t += U) {
+        unsigned v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __float_as_uint(lds_at(ar, qc_off(tab[s0 + t + u], rr, z4)));
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc ^= (t + u < dc) ? v[u] : 0u;
+    }
+    return acc >> 31;
+}
+
 // Whether every check of the lane's frame slot is satisfied.  `bits` is the
 // sign-bitmap area of frame slot 0 (slot f at bits + f * syn_words), `app`
 // the APP of slot 0 (slot f at app + f * N); `test` (the same on every lane of
@@ -348,9 +374,17 @@ __device__ __forceinline__ float with_sign(float mag, unsigned bits, int e) {
 // each task (slot, block row, 32 rows from r0) XORs, over the block row's
 // circulants {j, shift}, the 32 bits from column (r0 + shift) mod Z of
 // bitmap j: bit i of the result is the parity of row r0 + i.
+//
+// prechecked: the caller has already set s_flag of every tested slot to 1
+// (a row of the last block row of the iteration has odd hard-decision
+// parity: rejected) or 0 (all its rows even).  The other block rows are then
+// checked row by row straight from the APP, kSynGroup block rows between
+// barriers, last block rows first, until every tested slot is rejected or
+// all rows are even (no bitmaps: a frame that passes the last block row is
+// near convergence, and a failing row is usually found in the first group).
 __device__ __forceinline__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2* tab, const float* app,
                                unsigned* bits, const QcLane& l, bool lane_ok, bool test, int F,
-                               int* s_flag) {
+                               int* s_flag, bool prechecked = false) {
     constexpr int U = 8;  // loads in flight per lane
     const int Z = c.lifting;
     const int N = c.num_variables;
@@ -361,6 +395,22 @@ __device__ __forceinline__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2
     const int K = (Z + 31) >> 5;
     const int per_frame = c.base_rows * K;
     const int checks = F * per_frame;
+    if (prechecked) {
+        if (lane_ok && l.u == 0 && !test) s_flag[l.f] = 2;
+        const float* af = app + static_cast<size_t>(lane_ok ? l.f : 0) * N;
+        for (int br = c.base_rows - 2; br >= 0; br -= kSynGroup) {
+            if (!__syncthreads_or(lane_ok && l.u == 0 && s_flag[l.f] == 0)) break;
+            if (lane_ok && l.p == 0 && s_flag[l.f] == 0) {
+                unsigned odd = 0u;
+                for (int b = br; b >= 0 && b > br - kSynGroup; --b) odd |= qc_row_parity(c, tab, af, b, l.r);
+                if (odd) s_flag[l.f] = 1;
+            }
+        }
+        __syncthreads();
+        const bool ok = lane_ok ? (s_flag[l.f] == 0) : true;
+        __syncthreads();
+        return ok;
+    }
     if (lane_ok && l.u == 0) s_flag[l.f] = test ? 0 : 2;
     __syncthreads();
     // Quick rejection: lanes (f, r < 64, p = 0) check row r of block row 0
@@ -399,48 +449,45 @@ __device__ __forceinline__ bool qc_syndrome_ok(const ck_ldpc_code& c, const int2
         s1 = c.qc_row_ptr[br + 1];
     }
     {
-        // Chunk t of the F * nb * C chunks (slot f, column j, bits 64 ch ..
-        // 64 ch + 63, t = (f nb + j) C + ch) goes to words 2t, 2t + 1; lane i
-        // of its wave reads variable (64 ch + i) mod Z of column f nb + j,
-        // whose APP starts at app + (f nb + j) Z.  Wave w writes chunks
-        // [t0, t1); a group of up to eight chunks is stored by lanes 0..7.
+        // Chunk t = (f nb + j) C + ch (slot f, column j, bits 64 ch ..
+        // 64 ch + 63) goes to words 2t, 2t + 1: lane i's bit is the sign of
+        // variable (64 ch + i) mod Z of column f nb + j, whose APP starts at
+        // app + (f nb + j) Z.  Each wave takes whole columns (one ballot per
+        // chunk, kept by lane ch in a pair of registers), then lanes
+        // 0..C-1 store the column's C chunks.  Every lane is active here
+        // (the ballots count all 64).
         const int lane = threadIdx.x & 63;
         const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
         const int nw = static_cast<int>(blockDim.x >> 6);
-        const int T = F * nb * C;
-        const int tpw = (T + nw - 1) / nw;
-        const int t0 = min(T, wave * tpw);
-        const int t1 = min(T, t0 + tpw);
-        int fj = t0 / C;
-        int ch = t0 - fj * C;
-        const float* src = app + static_cast<size_t>(fj) * Z;
-        const int d = 64 % Z;   // column step from one chunk to the next
+        const int cols = F * nb;
+        const int d = 64 % Z;  // column step from one chunk to the next
         const int lm = lane % Z;
-        int col = (ch * 64 + lane) % Z;
-        for (int t = t0; t < t1; t += U) {
-            float v[U];
+        for (int fj = wave; fj < cols; fj += nw) {
+            const float* src = app + static_cast<size_t>(fj) * Z;
+            unsigned lo = 0u;
+            unsigned hi = 0u;
+            int col = lm;
+            for (int ch0 = 0; ch0 < C; ch0 += 4) {
+                float v[4];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                v[u] = *((t + u < t1) ? src + col : app);  // unconditional load
-                if (++ch == C) {
-                    ch = 0;
-                    src += Z;
-                    col = lm;
-                } else {
+                for (int u = 0; u < 4; ++u) {
+                    v[u] = src[col];  // col < Z: in the column whatever ch0 + u
                     col += d;
                     col -= (col >= Z) ? Z : 0;
                 }
-            }
-            unsigned lo = 0u;
-            unsigned hi = 0u;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const unsigned long long m = __ballot(v[u] < 0.0f);
-                lo = (lane == u) ? static_cast<unsigned>(m) : lo;
-                hi = (lane == u) ? static_cast<unsigned>(m >> 32) : hi;
+                for (int u = 0; u < 4; ++u) {
+                    if (ch0 + u < C) {
+                        const unsigned long long m = __ballot(v[u] < 0.0f);
+                        const bool mine = lane == ch0 + u;
+                        lo = mine ? static_cast<unsigned>(m) : lo;
+                        hi = mine ? static_cast<unsigned>(m >> 32) : hi;
+                    }
+                }
             }
-            if (lane < U && t + lane < t1)
-                *reinterpret_cast<uint2*>(bits + 2 * static_cast<size_t>(t + lane)) = make_uint2(lo, hi);
+            if (lane < C)
+                *reinterpret_cast<uint2*>(bits + 2 * (static_cast<size_t>(fj) * C + lane)) =
+                    make_uint2(lo, hi);
         }
     }
     __syncthreads();
@@ -523,10 +570,14 @@ constexpr int kSlotStep = 2;
 //     bit-reversed into edge order once per row;
 //   - R_e's sign bit is the row parity's flipped by Q_e's sign bit: the
 //     magnitude with the parity's sign bit, XORed with Q_e & 0x80000000.
-template <int P, int NT, bool UPDATE_APP>
+// PAR (with UPDATE_APP): *hard_parity = the parity of the row's hard
+// decisions on the updated APP (the XOR of their sign bits), the row's
+// syndrome bit once no later row of the iteration touches its bits.
+template <int P, int NT, bool UPDATE_APP, bool PAR = false>
 __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, const CheckState st,
                                                   int s0, int dc, int r, int p, int Z,
-                                                  const Params& prm) {
+                                                  const Params& prm,
+                                                  unsigned* hard_parity = nullptr) {
     constexpr int S = kSlotStep<P>;
     float q[NT];
     float* addr[NT];
@@ -593,6 +644,7 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
     // (mp ^ (Q & 0x80000000)) instead of the parity folded in per edge.
     asm volatile("" : "+v"(m1p), "+v"(m2p));
     int idx = -1;
+    unsigned hard = 0u;  // PAR: XOR of the updated APPs' bit patterns
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
@@ -603,16 +655,26 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
             const float mp = at_min ? m2p : m1p;
             const float rn = __uint_as_float(__float_as_uint(mp) ^
                                              (__float_as_uint(q[t]) & 0x80000000u));
-            *addr[t] = q[t] + rn;
+            const float an = q[t] + rn;
+            *addr[t] = an;
+            if (PAR) hard ^= __float_as_uint(an);
         }
     }
     if constexpr (P == 2) {
         const int2 vi = row_pair(idx);
         idx = max(vi.x, vi.y);
+        if (PAR) {
+            const int2 vh = row_pair(static_cast<int>(hard));
+            hard = static_cast<unsigned>(vh.x ^ vh.y);
+        }
     } else {
 #pragma unroll
-        for (int off = 1; off < P; off <<= 1) idx = max(idx, lane_xor(idx, off));
+        for (int off = 1; off < P; off <<= 1) {
+            idx = max(idx, lane_xor(idx, off));
+            if (PAR) hard ^= static_cast<unsigned>(lane_xor(static_cast<int>(hard), off));
+        }
     }
+    if (PAR) *hard_parity = hard >> 31;  // no APP is -0.0: sign bit = (APP < 0)
     CheckState out;
     out.m1 = m1;
     out.m2 = m2;
@@ -623,16 +685,21 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
 
 // Chooses the instantiation of qc_ms_row_n by the row's slot count, rounded
 // up to a multiple of kSlotStep (the count is uniform across the workgroup).
-template <int P, bool UPDATE_APP, int MAXT, int G = 1>
+template <int P, bool UPDATE_APP, int MAXT, int G = 1, bool PAR = false>
 __device__ __forceinline__ CheckState qc_ms_dispatch(int groups, const int2* tab, float* a,
                                                      const CheckState st, int s0, int dc, int r,
-                                                     int p, int Z, const Params& prm) {
+                                                     int p, int Z, const Params& prm,
+                                                     unsigned* hard_parity = nullptr) {
     constexpr int S = kSlotStep<P>;
     if constexpr (S * G >= MAXT) {
-        return qc_ms_row_n<P, S * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
+        return qc_ms_row_n<P, S * G, UPDATE_APP, PAR>(tab, a, st, s0, dc, r, p, Z, prm,
+                                                      hard_parity);
     } else {
-        if (groups <= G) return qc_ms_row_n<P, S * G, UPDATE_APP>(tab, a, st, s0, dc, r, p, Z, prm);
-        return qc_ms_dispatch<P, UPDATE_APP, MAXT, G + 1>(groups, tab, a, st, s0, dc, r, p, Z, prm);
+        if (groups <= G)
+            return qc_ms_row_n<P, S * G, UPDATE_APP, PAR>(tab, a, st, s0, dc, r, p, Z, prm,
+                                                          hard_parity);
+        return qc_ms_dispatch<P, UPDATE_APP, MAXT, G + 1, PAR>(groups, tab, a, st, s0, dc, r, p,
+                                                               Z, prm, hard_parity);
     }
 }
 
@@ -646,14 +713,16 @@ constexpr int kShortRowTwo = 22;
 
 // One min-sum check row `r` of block row `br`; MAXT bounds the lane's slots
 // (ceil(row degree / P)), which the host checks before choosing the kernel.
-template <int P, bool UPDATE_APP, int MAXT = kMaxDegree / P>
+template <int P, bool UPDATE_APP, int MAXT = kMaxDegree / P, bool PAR = false>
 __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int2* tab, float* a,
                                                 const CheckState st, int br, int r, int p,
-                                                const Params& prm) {
+                                                const Params& prm,
+                                                unsigned* hard_parity = nullptr) {
     const int s0 = ld_uniform(c.qc_row_ptr, br);
     const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
     const int groups = ((dc + P - 1) / P + kSlotStep<P> - 1) / kSlotStep<P>;
-    return qc_ms_dispatch<P, UPDATE_APP, MAXT>(groups, tab, a, st, s0, dc, r, p, c.lifting, prm);
+    return qc_ms_dispatch<P, UPDATE_APP, MAXT, 1, PAR>(groups, tab, a, st, s0, dc, r, p,
+                                                       c.lifting, prm, hard_parity);
 }
 
 // phi(x) = -ln(tanh(x / 2)) = ln((1 + e^-x) / (1 - e^-x)) for x clipped to
