@@ -178,6 +178,24 @@ class TestEncoder:
         u = np.random.default_rng(3).integers(0, 2, (3, code.k))
         assert not syndrome(code, encode_host(code, u)).any()
 
+    @pytest.mark.parametrize("name", list(CODES))
+    def test_row_weight_cap_rejects_no_draw(self, name):
+        """The 32-edge row cap never decides a seeded code: no draw up to the
+        accepted one has a row heavier than 32, so every code is the draw it
+        would be without the cap."""
+        from commkit_amd.workloads import _draw
+
+        spec = CODES[name]
+        rng = np.random.Generator(np.random.PCG64(spec.seed))
+        accepted = seeded_base(name)
+        for _ in range(10_000):
+            base = _draw(spec, rng)
+            assert (base >= 0).sum(axis=1).max() <= 32
+            if np.array_equal(base, accepted):
+                break
+        else:
+            pytest.fail("accepted base matrix not reached")
+
     def test_gf2(self):
         rng = np.random.default_rng(4)
         for _ in range(5):
