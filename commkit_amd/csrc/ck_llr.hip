@@ -4,27 +4,30 @@
 //                     the snapped level against the nearer of the two levels
 //                     just outside its run of equal bits (1 + 2 kb metrics);
 //   separable_levels  the metrics of each axis's levels with the log prior,
-//                     max-log or exact log-sum-exp: a unimodal Gray form
-//                     (one peak search, compile-time halves) for Gray
-//                     labels over equally spaced levels with a concave log
-//                     prior, a general form (argmax per half) otherwise;
+//                     max-log or exact log-sum-exp: unimodal Gray forms
+//                     (compile-time halves) for Gray labels over equally
+//                     spaced levels with a concave log prior, a general form
+//                     (argmax per half) otherwise;
 //   points            the metrics of all M points (any constellation).
 //
 // Semantics follow commkit/mapping/_numba_llr.py:35-235 and
-// commkit/_cuda/src/llr.cu:100-337 (same routing, same exact-LSE floor of
-// 1e-26 below which a half is summed relative to its own maximum).  The
-// arithmetic is this library's own, chosen for accuracy against an FP64
-// evaluation of the definition:
+// commkit/_cuda/src/llr.cu:100-337 (same routing; a half whose sum would
+// lose precision is summed relative to its own maximum).  The arithmetic is
+// this library's own, chosen for accuracy against an FP64 evaluation of the
+// definition:
 //
-//   - a difference of two metrics m_a - m_c is never formed by subtracting
-//     the metrics; it is evaluated in factored form,
+//   - a difference of two metrics is never formed in float32 from two large
+//     metrics: max-log takes it in float64, in factored form,
 //       m_a - m_c = (l_a - l_c) * ((x - l_a) + (x - l_c)) * inv_s2
 //                   + (logp_a - logp_c),
 //     so an LLR near zero (x near a decision boundary) keeps its relative
-//     accuracy instead of cancelling two large metrics;
-//   - the exact sums take exp(m - m_peak) with that factored difference, one
-//     exponential per level (or point; the hardware exp2 in the Gray form,
-//     ocml's exp2f elsewhere), and the two log(1 + T) terms of a bit as one
+//     accuracy;
+//   - the exact Gray form takes every level's metric in float64 relative to
+//     one reference level, its exponential as 2^n (exact) times a hardware
+//     exp2 of a fraction in [-1/2, 1/2], the half sums in float64, and each
+//     LLR as one oriented log(1 + q) of the two half sums (log_ratio); the
+//     general and point forms take exp(m - m_peak) per level or point with
+//     the factored difference and the two log(1 + T) terms of a bit as one
 //     logf (log1p_diff).
 //
 // Layout: per axis, the tables (levels, log prior, labels, opposite levels,
@@ -106,6 +109,9 @@ __device__ __forceinline__ void store_tile_perm(const float* tile, float* __rest
     }
 }
 
+constexpr double kLog2eD = 1.4426950408889634;
+constexpr float kLn2 = 0.6931471805599453f;
+
 // LDS copy of one axis with 2^KB levels: float64 levels and log prior for
 // the metric differences, float32 copies for locating maxima.
 template <int KB>
@@ -113,6 +119,7 @@ struct AxisTab {
     static constexpr int L = 1 << KB;
     double lev[L];
     double logp[L];
+    double logp2[L];  // log prior in log2 units (exact unimodal Gray form)
     float levf[L];
     float logpf[L];
     int lab[L];
@@ -138,6 +145,7 @@ __device__ void stage_axis(AxisTab<KB>& t, const ck_axis& a, bool with_opposite)
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
         t.lev[i] = a.levels[i];
         t.logp[i] = a.log_prior[i];
+        t.logp2[i] = a.log_prior[i] * kLog2eD;
         t.levf[i] = static_cast<float>(a.levels[i]);
         t.logpf[i] = static_cast<float>(a.log_prior[i]);
         t.lab[i] = a.labels[i];
@@ -249,12 +257,6 @@ __device__ __forceinline__ double diff_to(double la, double pa, double xd, doubl
     return (la - lc) * ((xd - la) + xc) * inv_s2 + (pa - pc);
 }
 
-// Below this (in nats) under the peak, the other half's best level is summed
-// relative to itself.  Hardware exp2 may flush a term more than 126 octaves
-// under the peak to zero; against a best term above exp(-60) such a term is
-// under exp(-27) of the half, so flushing it changes nothing measurable.
-constexpr double kWeakGray = -60.0;
-
 // log(1 + a) - log(1 + b) for a, b >= 0 as one logarithm: oriented so that
 // q = (max - min) / (1 + min) >= 0, and log(1 + q) = log(u) + r / u with
 // u + r = 1 + q exactly (TwoSum), so the result carries about one rounding
@@ -272,21 +274,15 @@ __device__ __forceinline__ float log1p_diff(float a, float b) {
     return neg ? -d : d;
 }
 
-// LLRs of one axis whose tables are flagged unimodal_gray (levels ascending
-// and equally spaced, log prior concave, labels Gray up to a constant XOR).
-// The float32 metrics locate the peak level pk; for each bit the half that
-// holds pk has pk as its best level, and the other half's best level a_o is
-// the better, in float64 metric, of pk's two nearest levels of opposite bit.
-// Max-log: LLR = -/+ (m_o - m_pk).  Exact: every other level's term is
-// exp(m_l - m_pk); T_own sums pk's half without pk, U_oth the other half
-// without a_o, and LLR = (m_pk - m_o) + log(1 + T_own) - log(1 + U_oth / e_o)
-// with e_o = exp(m_o - m_pk), sign by the half of pk: the float64 difference
-// and log(1 + T) terms of the generic form, the two logarithms taken as one
-// by log1p_diff.  Half membership is compile-time: Gray bit b of level l is
-// a constant of the unrolled loops.
+// Max-log LLRs of one axis whose tables are flagged unimodal_gray (levels
+// ascending and equally spaced, log prior concave, labels Gray up to a
+// constant XOR).  The float32 metrics locate the peak level pk; for each bit
+// the half that holds pk has pk as its best level, and the other half's best
+// level a_o is the better, in float64 metric, of pk's two nearest levels of
+// opposite bit: LLR = -/+ (m_o - m_pk), sign by the half of pk.
 template <int KB>
-__device__ __forceinline__ void axis_levels_gray(const AxisTab<KB>& t, float x, float inv_s2f,
-                                                 double inv_s2, bool exact, float* row) {
+__device__ __forceinline__ void axis_maxlog_gray(const AxisTab<KB>& t, float x, float inv_s2f,
+                                                 double inv_s2, float* row) {
     constexpr int L = 1 << KB;
     float best = -ck::kInf;
     int pk = 0;
@@ -304,76 +300,169 @@ __device__ __forceinline__ void axis_levels_gray(const AxisTab<KB>& t, float x, 
     const double ppk = t.logp[pk];
     const double xpk = xd - lpk;
     const int gpk = gray_of(pk);
-    int ao[KB];
-    double wo[KB];
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
+        const int sh = KB - 1 - b;
         int jl, jh;
-        gray_opposite<L>(pk, KB - 1 - b, &jl, &jh);
+        gray_opposite<L>(pk, sh, &jl, &jh);
         double wl = -1.0e300;
         double wh = -1.0e300;
         if (jl >= 0) wl = diff_to(t.lev[jl], t.logp[jl], xd, lpk, ppk, xpk, inv_s2);
         if (jh >= 0) wh = diff_to(t.lev[jh], t.logp[jh], xd, lpk, ppk, xpk, inv_s2);
-        ao[b] = (wh > wl) ? jh : jl;
-        wo[b] = (wh > wl) ? wh : wl;
+        const float r = static_cast<float>(-((wh > wl) ? wh : wl));
+        row[t.pos[b]] = ((((gpk ^ t.gray_xor) >> sh) & 1) != 0) ? -r : r;
     }
-    if (!exact) {
+}
+
+// Below this half sum (relative to the reference level's term 1) the half
+// is summed again relative to its own best level: its float64 terms would
+// lose precision to subnormals.
+constexpr double kFarHalf = 0x1p-1000;
+
+// The metric of level l in log2 units: (log p_l - (x - l)^2 / sigma^2) log2(e).
+template <int KB>
+__device__ __forceinline__ double metric2(const AxisTab<KB>& t, double xd, double s2l, int l) {
+    const double d = xd - t.lev[l];
+    return fma(-s2l * d, d, t.logp2[l]);
+}
+
+// 2^w in float64 for a float64 w: 2^n (exact, by ldexp) times 2^f by the
+// hardware exp2 with n = rint(w) and f = w - n in [-1/2, 1/2], so the term
+// carries the rounding of exp2 and of f (under 2^-25 absolute) whatever |w|,
+// and stays a normal float64 down to 2^-1022.
+__device__ __forceinline__ double exp2_split(double w) {
+    const double n = __builtin_rint(w);
+    const float f = static_cast<float>(w - n);
+    return __builtin_ldexp(static_cast<double>(__builtin_amdgcn_exp2f(f)), static_cast<int>(n));
+}
+
+// log(x) of a positive float64 from its binary exponent and the float32
+// log2 of its mantissa: relative error a few float32 roundings, for any
+// normal x.
+__device__ __forceinline__ float log_of(double x) {
+    int e;
+    const double m = frexp(x, &e);
+    return (static_cast<float>(e) + __builtin_amdgcn_logf(static_cast<float>(m))) * kLn2;
+}
+
+// log(S_a / S_b) for S_a, S_b > 0 given in float64.  For a ratio under 2^64
+// it is oriented so that q = |S_a - S_b| / min(S_a, S_b) >= 0 (formed in
+// float64, then rounded once) and log(1 + q) = log(u) + r / u with u + r =
+// 1 + q exactly (TwoSum): an LLR near zero keeps an absolute error of about
+// one float32 rounding of q, and u >= 1 needs no range reduction for the
+// hardware log2.  A larger ratio (an LLR above 44) is the difference of the
+// two logarithms.
+__device__ __forceinline__ float log_ratio(double sa, double sb) {
+    const double d = sa - sb;
+    const double lo = d < 0.0 ? sa : sb;
+    const double hi = d < 0.0 ? sb : sa;
+    float v;
+    if (lo >= 0x1p-64 * hi) {
+        double rl = __builtin_amdgcn_rcp(lo);
+        rl = fma(rl, fma(-lo, rl, 1.0), rl);  // one Newton step
+        const float q = static_cast<float>(fabs(d) * rl);
+        const float u = 1.0f + q;
+        const float bv = u - 1.0f;
+        const float r = (1.0f - (u - bv)) + (q - bv);
+        v = __builtin_amdgcn_logf(u) * kLn2 + r * __builtin_amdgcn_rcpf(u);
+    } else {
+        v = log_of(hi) - log_of(lo);
+    }
+    return d < 0.0 ? -v : v;
+}
+
+// Exact (log-sum-exp) LLRs of one unimodal Gray axis (conditions as for
+// axis_maxlog_gray).  Every level's metric is taken in float64 relative to
+// a reference level s (the snapped level of x), in log2 units, w_l =
+// M_l - M_s; its term 2^w_l (exp2_split) is added in float64 to one of two
+// half sums per bit, the half being the level's Gray bit, a compile-time
+// constant of the unrolled loop.  Bit b's LLR is log(S_b0 / S_b1) by
+// log_ratio, negated where the label XOR flips the bit.
+//
+// The half holding s has a sum >= 1 (s's own term).  When the other half's
+// sum falls under kFarHalf, every level between s and the peak pk has a
+// metric >= M_s (the metric is concave in the level index), so s and pk lie
+// in one run of equal bits, the other half's best level a_o is the better of
+// s's two nearest levels of opposite bit (gray_opposite), and that half is
+// summed again relative to a_o:  log S_oth = (w_o + log2(1 + U)) ln 2 with
+// U = sum over the other levels of the half of 2^(M_l - M_o).
+//
+// A prior far more peaked than the likelihood can put a level 2^996 above s
+// (sums over 1e300): then the sums are taken again with s = pk, located in
+// float64.
+template <int KB>
+__device__ __forceinline__ void axis_exact_gray(const AxisTab<KB>& t, float x, double inv_s2,
+                                                float inv_step, float* row) {
+    constexpr int L = 1 << KB;
+    const double xd = x;
+    const double s2l = inv_s2 * kLog2eD;
+    int s = __float2int_rn((x - t.levf[0]) * inv_step);
+    s = min(max(s, 0), L - 1);
+    double S0[KB], S1[KB];
+    double ms = 0.0;
+    for (int pass = 0;; ++pass) {
+        ms = metric2(t, xd, s2l, s);
 #pragma unroll
         for (int b = 0; b < KB; ++b) {
-            const int sh = KB - 1 - b;
-            const float r = static_cast<float>(-wo[b]);
-            row[t.pos[b]] = ((((gpk ^ t.gray_xor) >> sh) & 1) != 0) ? -r : r;
+            S0[b] = 0.0;
+            S1[b] = 0.0;
         }
-        return;
-    }
-    float s0[KB], s1[KB];
 #pragma unroll
-    for (int b = 0; b < KB; ++b) {
-        s0[b] = 0.0f;
-        s1[b] = 0.0f;
-    }
+        for (int l = 0; l < L; ++l) {
+            const double d = xd - t.lev[l];
+            const double e = exp2_split(fma(-s2l * d, d, t.logp2[l] - ms));
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
-        const double w = diff_to(t.lev[l], t.logp[l], xd, lpk, ppk, xpk, inv_s2);
-        float e = __builtin_amdgcn_exp2f(static_cast<float>(w) * kLog2e);
-        e = (l == pk) ? 0.0f : e;
-#pragma unroll
-        for (int b = 0; b < KB; ++b) {
-            const float eb = (l == ao[b]) ? 0.0f : e;
-            if ((gray_of(l) >> (KB - 1 - b)) & 1)
-                s1[b] += eb;
-            else
-                s0[b] += eb;
+            for (int b = 0; b < KB; ++b) {
+                if ((gray_of(l) >> (KB - 1 - b)) & 1)
+                    S1[b] += e;
+                else
+                    S0[b] += e;
+            }
+            // Four levels at a time: the scheduler would otherwise issue
+            // every level's table reads up front and spill.
+            if ((l & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        // Finite sums (also the NaN symbol's second pass) end the loop.
+        if (S0[0] + S1[0] < 1.0e300 || pass == 1) break;
+        double best = -1.0e300;
+#pragma unroll 1
+        for (int l = 0; l < L; ++l) {
+            const double m = metric2(t, xd, s2l, l);
+            if (m > best) {
+                best = m;
+                s = l;
+            }
         }
     }
+    const int gs = gray_of(s);
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
         const int sh = KB - 1 - b;
-        const int own = (gpk >> sh) & 1;
-        const float t_own = own ? s1[b] : s0[b];
-        float t_oth;
-        if (wo[b] >= kWeakGray) {
-            t_oth = (own ? s0[b] : s1[b]) /
-                    __builtin_amdgcn_exp2f(static_cast<float>(wo[b]) * kLog2e);
+        const int own = (gs >> sh) & 1;
+        const double s_own = own ? S1[b] : S0[b];
+        const double s_oth = own ? S0[b] : S1[b];
+        float v;  // log(S_own / S_oth)
+        if (s_oth >= kFarHalf) {
+            v = log_ratio(s_own, s_oth);
         } else {
-            // The other half lies far under the peak: its terms relative to
-            // its own best level a_o.
-            const int a = ao[b];
-            const double la = t.lev[a];
-            const double pa = t.logp[a];
-            const double xa = xd - la;
-            t_oth = 0.0f;
+            int jl, jh;
+            gray_opposite<L>(s, sh, &jl, &jh);
+            const double ml = jl >= 0 ? metric2(t, xd, s2l, jl) : -1.0e300;
+            const double mh = jh >= 0 ? metric2(t, xd, s2l, jh) : -1.0e300;
+            const int a = mh > ml ? jh : jl;
+            const double mo = mh > ml ? mh : ml;
+            float u = 0.0f;
 #pragma unroll 1
             for (int l = 0; l < L; ++l) {
                 if (((gray_of(l) >> sh) & 1) != own && l != a)
-                    t_oth += exp2f(
-                        static_cast<float>(diff_to(t.lev[l], t.logp[l], xd, la, pa, xa, inv_s2)) *
-                        kLog2e);
+                    u += __builtin_amdgcn_exp2f(static_cast<float>(metric2(t, xd, s2l, l) - mo));
             }
+            // log S_own - log S_oth, with S_oth = 2^(mo - ms) (1 + u)
+            const double lo = (ms - mo) * 0.6931471805599453 +
+                              static_cast<double>(log_of(s_own) - log1pf(u));
+            v = static_cast<float>(lo);
         }
-        const double v = -wo[b] + static_cast<double>(log1p_diff(t_own, t_oth));
-        const float r = static_cast<float>(v);
-        row[t.pos[b]] = ((own ^ (t.gray_xor >> sh)) & 1) ? -r : r;
+        row[t.pos[b]] = ((own ^ (t.gray_xor >> sh)) & 1) ? -v : v;
     }
 }
 
@@ -387,10 +476,14 @@ __device__ __forceinline__ void axis_levels_gray(const AxisTab<KB>& t, float x, 
 // maximum directly).
 template <int KB>
 __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float inv_s2f,
-                                            double inv_s2, bool exact, float* row) {
+                                            double inv_s2, float inv_step, bool exact,
+                                            float* row) {
     if constexpr (KB <= 5) {
         if (t.unimodal_gray) {
-            axis_levels_gray<KB>(t, x, inv_s2f, inv_s2, exact, row);
+            if (exact)
+                axis_exact_gray<KB>(t, x, inv_s2, inv_step, row);
+            else
+                axis_maxlog_gray<KB>(t, x, inv_s2f, inv_s2, row);
             return;
         }
     }
@@ -531,8 +624,14 @@ __device__ __forceinline__ void separable_body(CK_SEPARABLE_PARAMS) {
                 axis_closed<KB>(tab[0], v.x, inv_s2, inv_step0, row);
                 axis_closed<KB>(tab[1], v.y, inv_s2, inv_step1, row);
             } else {
-                axis_levels<KB>(tab[0], v.x, inv_s2f, inv_s2, exact != 0, row);
-                axis_levels<KB>(tab[1], v.y, inv_s2f, inv_s2, exact != 0, row);
+                // The table index is opaque to the compiler, so the level
+                // tables are read from LDS per symbol (broadcast reads)
+                // instead of held in registers across the loop: 2 x 2^KB
+                // float64 values per axis would spill.
+                int z = 0;
+                asm volatile("" : "+v"(z));
+                axis_levels<KB>(tab[z], v.x, inv_s2f, inv_s2, inv_step0, exact != 0, row);
+                axis_levels<KB>(tab[z + 1], v.y, inv_s2f, inv_s2, inv_step1, exact != 0, row);
             }
         }
         __syncthreads();

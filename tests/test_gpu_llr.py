@@ -269,11 +269,12 @@ def _natural_and_bimodal():
 
 
 GRAY_SWEEP = {
-    "16qam": (lambda: Constellation.qam(16), [5.0, 10.0, 20.0, 30.0, 40.0]),
-    "64qam": (lambda: Constellation.qam(64), [10.0, 15.0, 20.0, 30.0, 40.0]),
-    "256qam": (lambda: Constellation.qam(256), [10.0, 15.0, 20.0, 30.0, 40.0]),
-    "256qam-shaped": (lambda: Constellation.qam(256).shaped(nu=0.05), [10.0, 13.7, 20.0, 30.0]),
-    "1024qam": (lambda: Constellation.qam(1024), [25.0, 30.0, 40.0]),
+    "16qam": (lambda: Constellation.qam(16), [-3.0, 5.0, 10.0, 20.0, 30.0, 40.0]),
+    "64qam": (lambda: Constellation.qam(64), [-3.0, 3.0, 10.0, 15.0, 20.0, 30.0, 40.0]),
+    "256qam": (lambda: Constellation.qam(256), [-3.0, 3.0, 8.0, 15.0, 20.0, 30.0, 40.0]),
+    "256qam-shaped": (lambda: Constellation.qam(256).shaped(nu=0.05),
+                      [-3.0, 0.0, 3.0, 8.0, 13.7, 20.0, 30.0]),
+    "1024qam": (lambda: Constellation.qam(1024), [-3.0, 3.0, 8.0, 13.7, 25.0, 30.0, 40.0]),
     "16qam-natural": (lambda: _natural_and_bimodal()[0], [5.0, 15.0, 30.0]),
     "16qam-bimodal-prior": (lambda: _natural_and_bimodal()[1], [5.0, 15.0, 30.0]),
 }
@@ -296,7 +297,7 @@ def test_llr_sweep_against_fp64(gpu, name, method):
                     "test_llr_against_fp64 holds to FP64 for QPSK to 1024-QAM")  # fmt: skip
     for db in dbs:
         nv = 10 ** (-db / 10)
-        x = _noisy(c, 4096 if c.order >= 1024 else 8192, nv, seed=int(db * 10) + 7)
+        x = _noisy(c, 4096 if c.order >= 1024 else 8192, nv, seed=int(db * 10) + 107)
         got = compute_llr(torch.from_numpy(x).cuda(), noise_var=nv, constellation=c,
                           method=method).cpu().numpy()  # fmt: skip
         ref = llr_fp64(x, c.points, c.bit_labels, c.pmf, nv, method)
@@ -305,3 +306,52 @@ def test_llr_sweep_against_fp64(gpu, name, method):
         print(f"{name} {method} {db} dB: worst error / bar {rel.max() / 1e-5:.3f}")
         assert ok, f"{db} dB: worst error {rel.max() / 1e-5:.3f} x the bar"
         assert np.array_equal(np.sign(got[~under]), np.sign(ref[~under]))
+
+
+def test_prior_steeper_than_the_likelihood(gpu):
+    """A log prior far steeper than the likelihood (-1000 (i - 3/2)^2 nats
+    over 16-QAM's axis levels: a symbol snapped to an edge level has a level
+    about 2000 nats above it, beyond the float64 range of the half sums)
+    takes the second pass with the peak as reference.  Such a prior lies
+    below the Python API's 1e-40 clip, so the tables go through the C-ABI;
+    the expected LLRs are the definition in float64 over the 16 points."""
+    import ctypes
+
+    import torch
+
+    from commkit_amd import _native
+    from commkit_amd.mapping.demap import CkAxis, _axis_tables
+    from commkit_amd.mapping.lattice import lattice_coordinates
+
+    c = Constellation.qam(16)
+    lp = -1000.0 * (np.arange(4) - 1.5) ** 2
+    keep, axes = [], []
+    for a in c._axes:
+        t = list(_axis_tables(a))
+        t[1] = lp.copy()
+        t = [torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in t]
+        keep.append(t)
+        axes.append(CkAxis(*(v.data_ptr() for v in t), int(t[0].numel()), int(t[5].numel())))
+    nv = 1.0
+    x = _noisy(c, 4096, nv, seed=11)
+    x[:64] = 1.2 * np.exp(2j * np.pi * np.arange(64) / 64)  # snapped to edge levels
+    xt = torch.from_numpy(x.astype(np.complex64)).cuda()
+    out = torch.empty((x.size, c.bits_per_symbol), dtype=torch.float32, device="cuda")
+    lib = _native.lib()
+    _native.check(lib.ck_llr_separable(
+        xt.data_ptr(), x.size, ctypes.byref(axes[0]), ctypes.byref(axes[1]),
+        ctypes.c_double(1.0 / nv), 1, out.data_ptr(), _native.stream_of(xt.device)))  # fmt: skip
+    got = out.cpu().numpy().reshape(-1)
+    side, where = lattice_coordinates(c.points)
+    lp_pt = np.empty(c.order)
+    for i in range(side):
+        for q in range(side):
+            lp_pt[where[i, q]] = lp[i] + lp[q]
+    xs = x.astype(np.complex64).astype(np.complex128)
+    metric = -np.abs(xs[:, None] - c.points[None, :]) ** 2 / nv + lp_pt[None, :]
+    ref = np.stack([np.logaddexp.reduce(metric[:, c.bit_labels[:, b] == 0], axis=1) -
+                    np.logaddexp.reduce(metric[:, c.bit_labels[:, b] == 1], axis=1)
+                    for b in range(c.bits_per_symbol)], axis=1).reshape(-1)  # fmt: skip
+    assert np.isfinite(got).all()
+    ok, _ = llr_tolerance_ok(got, ref, floor=FLOOR["exact"])
+    assert ok, f"worst error {np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 2**-5)):.3e}"
