@@ -61,20 +61,24 @@ LEGS = ("c1", "c3", "c4", "c5")
 # Counters cannot be read inside this process; --traffic reads them live in
 # child processes under rocprofv3 instead.
 PROFILED_TRAFFIC = {
-    "c2": {"frames": FRAMES, "bytes": 2.320e9,
+    "c2": {"frames": FRAMES, "bytes": 2.270e9, "demap_bytes": 0.393e9,
            "profile": "profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md"},
-    "c5": {"frames": 3106, "bytes": 8.415e9,
+    "c5": {"frames": 3106, "bytes": 8.414e9, "demap_bytes": 0.3366e9,
            "profile": "profiles/ldpc_sum_product_shaped256_3106cw.md"},
 }  # fmt: skip
 
 # What the counters of the decoder kernels show limits them (the profiles
 # above): VALU issue and waits, not HBM bandwidth.
 LIMITER = {
-    "c2": "VALU issue and barrier/LDS waits (SQ_INSTS_VALU fills ~59% of SIMD cycles, "
-          "SQ_WAIT_ANY 46%); APP and check state live in LDS, so HBM traffic is a fifth "
-          "of the algorithmic bytes",
-    "c5": "HBM latency of the per-edge message loads at row start (float32 messages in "
-          "HBM) and transcendental issue",
+    "c2": "VALU issue and barrier/LDS waits (SQ_INSTS_VALU fills ~54% of SIMD cycles, "
+          "SQ_WAIT_ANY 47% of wave cycles); APP and check state live in LDS, so HBM "
+          "traffic is a fifth of the algorithmic bytes",
+    "c5": "VALU issue, the phi evaluations first (SQ_INSTS_VALU fills ~66% of SIMD cycles, "
+          "73% by SQ_ACTIVE_INST_VALU; 6.5 transcendentals per edge and iteration)",
+    "c2_demap": "VALU issue (~70% of SIMD cycles) and load latency (60% of wave cycles "
+                "waiting); a device copy of the same bytes runs at 5.3 TB/s",
+    "c5_demap": "VALU issue (~70% of SIMD cycles, 1017 instructions per symbol: float64 level "
+                "metrics, one exp2 per level)",
 }  # fmt: skip
 
 
@@ -518,7 +522,9 @@ def run_single(args, dev) -> tuple[dict, bool]:
         "kernels_ms": {"demap": round(demap_ms, 4), "decode": round(decode_ms, 4)},
         "roofline": roofline(dec_bytes, decode_ms, "ldpc_qc_layered (decode)", traffic=traffic,
                              source=prof["profile"], limiter=LIMITER["c2"]),  # fmt: skip
-        "demap_roofline": roofline(demap_bytes, demap_ms, "llr_separable_kernel (demap)"),
+        "demap_roofline": roofline(demap_bytes, demap_ms, "llr_separable_kernel (demap)",
+                                   traffic=prof["demap_bytes"] if traffic is not None else None,
+                                   source=prof["profile"], limiter=LIMITER["c2_demap"]),
         "tail_roofline": {
             "achieved": round((demap_bytes + dec_bytes) / ((demap_ms + decode_ms) * 1e-3) / 1e9, 1),
             "peak": HBM_PEAK_GBS,
@@ -891,7 +897,9 @@ def run_shaped(args, dev) -> tuple[dict, bool]:
                              traffic=prof["bytes"], source=prof["profile"],
                              limiter=LIMITER["c5"]),  # fmt: skip
         "demap_roofline": roofline(demap_bytes, demap_ms,
-                                   "llr_separable_kernel_w4<4,false> (exact demap, permuted store)"),
+                                   "llr_separable_kernel_w4<4,false> (exact demap, permuted store)",
+                                   traffic=prof["demap_bytes"], source=prof["profile"],
+                                   limiter=LIMITER["c5_demap"]),
     }
     progress(leg="c5", value=out["value"], decode_ms=round(decode_ms, 4))
     # Parity: the oracle decodes the GPU LLRs of the first 64 codewords
