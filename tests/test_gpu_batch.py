@@ -4,7 +4,8 @@ Every frame of a batch must decode exactly as the per-call decoder decodes
 it alone (which the per-call tests hold bit-exact to the CPU oracle for
 min-sum): bits, APP LLRs, iteration counts and valid flags, in direct
 launches and in replays of the captured HIP graph, with other GPU work
-between the replays.
+between the replays.  The mixed workload's batched launch is also held to
+the CPU oracle directly.
 """
 
 import numpy as np
@@ -16,6 +17,7 @@ from commkit_amd.coding.ldpc import _decode
 from commkit_amd.workloads import seeded_code
 
 from .codes import encode_host
+from .test_gpu_ldpc import oracle_decode
 
 pytestmark = pytest.mark.gpu
 
@@ -90,14 +92,16 @@ def test_batch_early_exit_and_empty_class(gpu):
 def test_mixed_workload_batch_equals_per_code(gpu):
     """The mixed workload (three codes, four QAM orders, per-frame Es/N0),
     demapped per group into the code classes and decoded in one batched
-    launch, equals decoding each code's frames alone."""
+    launch with the mixed benchmark's method (layered NMS 0.875, <= 50
+    iterations), equals decoding each code's frames alone, and equals the
+    CPU oracle decoding the same LLRs on every frame."""
     from commkit_amd import Constellation
     from commkit_amd.mapping.demap import llr_rows
     from commkit_amd.workloads import MIXED_ORDERS, make_mixed
 
     mx = make_mixed(240, seed=5)
     assert sum(mx.rows) == 240 and len(mx.groups) == 12
-    method = MinSum(max_iterations=30)
+    method = MinSum(max_iterations=50)
     bd = BatchDecoder(mx.codes, mx.rows, method)
     consts = {o: Constellation.qam(o) for o in MIXED_ORDERS}
     for g in mx.groups:
@@ -106,6 +110,9 @@ def test_mixed_workload_batch_equals_per_code(gpu):
     bd.run()
     for i, c in enumerate(mx.codes):
         _assert_class(bd.outputs[i], _alone(c, bd.inputs[i], method), f"code {i}")
+        want = oracle_decode(c, bd.inputs[i].cpu().numpy(), method)
+        _assert_class(bd.outputs[i], [want[0], want[1], want[2].astype(np.int8), want[3]],
+                      f"code {i} vs oracle")
     ok = sum(int(bd.outputs[g.code_index].valid[g.row0:g.row1].sum()) for g in mx.groups)
     assert ok > 120  # most frames sit above their waterfall
     bd.close()
