@@ -22,13 +22,13 @@
 //                   + (logp_a - logp_c),
 //     so an LLR near zero (x near a decision boundary) keeps its relative
 //     accuracy;
-//   - the exact Gray form takes every level's metric in float64 relative to
-//     one reference level, its exponential as 2^n (exact) times a hardware
-//     exp2 of a fraction in [-1/2, 1/2], the half sums in float64, and each
-//     LLR as one oriented log(1 + q) of the two half sums (log_ratio); the
-//     general and point forms take exp(m - m_peak) per level or point with
-//     the factored difference and the two log(1 + T) terms of a bit as one
-//     logf (log1p_diff).
+//   - the exact Gray and point forms take every level's (point's) metric in
+//     float64 relative to one reference, its exponential as 2^n (exact)
+//     times a hardware exp2 of a fraction in [-1/2, 1/2], the half sums in
+//     float64, and each LLR as one oriented log(1 + q) of the two half sums
+//     (log_ratio); the general form takes exp(m - m_peak) per level with the
+//     factored difference and the two log(1 + T) terms of a bit as one logf
+//     (log1p_diff).
 //
 // Layout: per axis, the tables (levels, log prior, labels, opposite levels,
 // bit positions) are staged in LDS once per workgroup.  Each lane writes its
@@ -679,24 +679,44 @@ __global__ __launch_bounds__(kThreads) void llr_points_kernel(
     double inv_s2, int exact, float* __restrict__ out) {
     __shared__ float tile[kThreads * (K | 1)];
     extern __shared__ char dyn[];
-    const bool staged = M <= kPointsInLds;
+    // M = 2^K (checked by the host): the tables of up to kPointsInLds
+    // points are staged in LDS, decided at compile time so that the
+    // compiler reads them as LDS, not through generic pointers.
+    constexpr bool staged = (1 << K) <= kPointsInLds;
     const double2* pts = g_pts;
     const double* logp = g_logp;
+    const double* logp2 = nullptr;  // log prior in log2 units (staged only)
     const int* lab_tab = g_lab;
-    if (staged) {
+    if constexpr (staged) {
         double2* s_pts = reinterpret_cast<double2*>(dyn);
         double* s_logp = reinterpret_cast<double*>(s_pts + M);
-        int* s_lab = reinterpret_cast<int*>(s_logp + M);
+        double* s_logp2 = s_logp + M;
+        int* s_lab = reinterpret_cast<int*>(s_logp2 + M);
         for (int i = threadIdx.x; i < M; i += kThreads) {
             s_pts[i] = g_pts[i];
             s_logp[i] = g_logp[i];
+            s_logp2[i] = g_logp[i] * kLog2eD;
             s_lab[i] = g_lab[i];
         }
         pts = s_pts;
         logp = s_logp;
+        logp2 = s_logp2;
         lab_tab = s_lab;
     }
     __syncthreads();
+    const double s2l = inv_s2 * kLog2eD;
+    // The metric of point m in log2 units relative to a reference metric.
+    const auto metric2 = [&](double xr, double xi, int m, double ref) {
+        const double2 p = pts[m];
+        const double dr = xr - p.x;
+        const double di = xi - p.y;
+        double lp;
+        if constexpr (staged)
+            lp = logp2[m];
+        else
+            lp = logp[m] * kLog2eD;
+        return fma(-s2l, fma(dr, dr, di * di), lp - ref);
+    };
     float* row = tile + threadIdx.x * tile_stride(K);
     for (long long base = static_cast<long long>(blockIdx.x) * kThreads; base < n;
          base += static_cast<long long>(gridDim.x) * kThreads) {
@@ -736,75 +756,66 @@ __global__ __launch_bounds__(kThreads) void llr_points_kernel(
                 for (int b = 0; b < K; ++b)
                     row[b] = static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2, a0[b], a1[b]));
             } else {
-                float p0[K], p1[K];
-#pragma unroll
-                for (int b = 0; b < K; ++b) {
-                    p0[b] = -ck::kInf;
-                    p1[b] = -ck::kInf;
-                    a0[b] = 0;
-                    a1[b] = 0;
-                }
+                // The peak point (float32 metrics) is the reference: every
+                // term 2^w, w = M_m - M_pk in log2 units (float64, then
+                // exp2_split), goes to one of the two float64 half sums of
+                // each bit.  A point's label is the same for every lane, so
+                // the half of each bit is a uniform branch, not a select.
+                float best = -ck::kInf;
+                int pk = 0;
                 for (int m = 0; m < M; ++m) {
                     const double2 p = pts[m];
                     const float dr = v.x - static_cast<float>(p.x);
                     const float di = v.y - static_cast<float>(p.y);
                     const float met = -(dr * dr + di * di) * inv_s2f + static_cast<float>(logp[m]);
-                    const int lab = lab_tab[m];
-#pragma unroll
-                    for (int b = 0; b < K; ++b) {
-                        if ((lab >> (K - 1 - b)) & 1) {
-                            if (met > p1[b]) { p1[b] = met; a1[b] = m; }
-                        } else {
-                            if (met > p0[b]) { p0[b] = met; a0[b] = m; }
-                        }
+                    if (met > best) {
+                        best = met;
+                        pk = m;
                     }
                 }
-                const int pk = (p0[0] >= p1[0]) ? a0[0] : a1[0];
-                double w0[K], w1[K];
-                float u0[K], u1[K];
+                const double ref = metric2(xr, xi, pk, 0.0);
+                double S0[K], S1[K];
 #pragma unroll
                 for (int b = 0; b < K; ++b) {
-                    w0[b] = point_diff(pts, logp, xr, xi, inv_s2, a0[b], pk);
-                    w1[b] = point_diff(pts, logp, xr, xi, inv_s2, a1[b], pk);
-                    u0[b] = 0.0f;
-                    u1[b] = 0.0f;
+                    S0[b] = 0.0;
+                    S1[b] = 0.0;
                 }
                 for (int m = 0; m < M; ++m) {
-                    const float e = exp2f(
-                        static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2, m, pk)) * kLog2e);
-                    const int lab = lab_tab[m];
+                    const double e = exp2_split(metric2(xr, xi, m, ref));
+                    const int lab = __builtin_amdgcn_readfirstlane(lab_tab[m]);
 #pragma unroll
                     for (int b = 0; b < K; ++b) {
-                        if ((lab >> (K - 1 - b)) & 1) {
-                            u1[b] += (m == a1[b]) ? 0.0f : e;
-                        } else {
-                            u0[b] += (m == a0[b]) ? 0.0f : e;
-                        }
+                        // 0/1 weights chosen on the scalar unit: one FMA per
+                        // half (S + 1 e = S + e, S + 0 e = S exactly).
+                        const bool one = (lab >> (K - 1 - b)) & 1;
+                        S1[b] = fma(one ? 1.0 : 0.0, e, S1[b]);
+                        S0[b] = fma(one ? 0.0 : 1.0, e, S0[b]);
                     }
                 }
+                const int lpk = lab_tab[pk];
 #pragma unroll
                 for (int b = 0; b < K; ++b) {
-                    float t0, t1;
-                    if (w0[b] < kWeak) {
-                        t0 = 0.0f;
-                        for (int m = 0; m < M; ++m)
-                            if (!((lab_tab[m] >> (K - 1 - b)) & 1) && m != a0[b])
-                                t0 += exp2f(static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2,
-                                                                          m, a0[b])) * kLog2e);
+                    const int own = (lpk >> (K - 1 - b)) & 1;  // the peak's half: sum >= 1
+                    const double s_own = own ? S1[b] : S0[b];
+                    const double s_oth = own ? S0[b] : S1[b];
+                    float r;  // log(S_own / S_oth)
+                    if (s_oth >= kFarHalf) {
+                        r = log_ratio(s_own, s_oth);
                     } else {
-                        t0 = u0[b] / exp2f(static_cast<float>(w0[b]) * kLog2e);
-                    }
-                    if (w1[b] < kWeak) {
-                        t1 = 0.0f;
+                        // The other half lies more than 1000 octaves under
+                        // the peak: summed again relative to its own best.
+                        double mo = -1.0e300;
                         for (int m = 0; m < M; ++m)
-                            if (((lab_tab[m] >> (K - 1 - b)) & 1) && m != a1[b])
-                                t1 += exp2f(static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2,
-                                                                          m, a1[b])) * kLog2e);
-                    } else {
-                        t1 = u1[b] / exp2f(static_cast<float>(w1[b]) * kLog2e);
+                            if (((lab_tab[m] >> (K - 1 - b)) & 1) != own)
+                                mo = fmax(mo, metric2(xr, xi, m, ref));
+                        double u = 0.0;
+                        for (int m = 0; m < M; ++m)
+                            if (((lab_tab[m] >> (K - 1 - b)) & 1) != own)
+                                u += exp2_split(metric2(xr, xi, m, ref) - mo);
+                        r = static_cast<float>(-mo * 0.6931471805599453 +
+                                               static_cast<double>(log_of(s_own) - log_of(u)));
                     }
-                    const double d01 = point_diff(pts, logp, xr, xi, inv_s2, a0[b], a1[b]);
-                    row[b] = static_cast<float>(d01 + static_cast<double>(log1p_diff(t0, t1)));
+                    row[b] = own ? -r : r;
                 }
             }
         }
@@ -946,7 +957,7 @@ extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,
     if (n == 0) return CK_OK;
     CK_REQUIRE(x && out && points && log_prior && labels, "NULL pointer argument");
     const int grid = grid_for(n);
-    const size_t dyn = M <= kPointsInLds ? static_cast<size_t>(M) * 28 : 0;
+    const size_t dyn = M <= kPointsInLds ? static_cast<size_t>(M) * 36 : 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const float2* xs = static_cast<const float2*>(x);
     const double2* pts = static_cast<const double2*>(points);

@@ -275,24 +275,28 @@ GRAY_SWEEP = {
     "256qam-shaped": (lambda: Constellation.qam(256).shaped(nu=0.05),
                       [-3.0, 0.0, 3.0, 8.0, 13.7, 20.0, 30.0]),
     "1024qam": (lambda: Constellation.qam(1024), [-3.0, 3.0, 8.0, 13.7, 25.0, 30.0, 40.0]),
-    "16qam-natural": (lambda: _natural_and_bimodal()[0], [5.0, 15.0, 30.0]),
-    "16qam-bimodal-prior": (lambda: _natural_and_bimodal()[1], [5.0, 15.0, 30.0]),
+    "16qam-natural": (lambda: _natural_and_bimodal()[0], [-3.0, 5.0, 15.0, 30.0]),
+    "16qam-bimodal-prior": (lambda: _natural_and_bimodal()[1], [-3.0, 5.0, 15.0, 30.0]),
+    "16psk": (lambda: Constellation.psk(16), [-3.0, 3.0, 10.0, 20.0, 30.0]),
+    "32qam-cross": (lambda: Constellation.qam(32), [-3.0, 3.0, 10.0, 20.0, 30.0]),
+    "128qam-cross": (lambda: Constellation.qam(128), [-3.0, 3.0, 10.0, 20.0, 30.0]),
 }
 
 
 @pytest.mark.parametrize("name", list(GRAY_SWEEP))
 @pytest.mark.parametrize("method", ["maxlog", "exact"])
 def test_llr_sweep_against_fp64(gpu, name, method):
-    """Separable level metrics over an Es/N0 sweep: Gray axes with a concave
-    (uniform or Maxwell-Boltzmann) log prior take the unimodal Gray form,
+    """Level and point metrics over an Es/N0 sweep: Gray axes with a concave
+    (uniform or Maxwell-Boltzmann) log prior take the unimodal Gray forms,
     whose far-under halves (high Es/N0) are summed relative to their own best
-    level; the natural-label and two-peak-prior cases take the general form.
-    Same tolerance as above on every element."""
+    level; the natural-label and two-peak-prior cases take the general form;
+    PSK and cross QAM the point form.  Same tolerance as above on every
+    element."""
     import torch
 
     make, dbs = GRAY_SWEEP[name]
     c = make()
-    if method == "maxlog" and c.pmf is None and name != "16qam-natural":
+    if method == "maxlog" and c.pmf is None and c._axes is not None and name != "16qam-natural":
         pytest.skip("uniform Gray max-log takes the closed form, which "
                     "test_llr_against_fp64 holds to FP64 for QPSK to 1024-QAM")  # fmt: skip
     for db in dbs:
