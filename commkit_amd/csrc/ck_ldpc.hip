@@ -9,11 +9,11 @@
 // float32 sum of phi(|Q|) with phi(x) = -log tanh(x/2), x clipped to
 // [1e-7, 30].  Pass 2 writes R_new = sign * mag and APP = Q + R_new, with
 // mag = max(scale * min - offset, 0) (min2 for edge idx) or
-// phi(total - phi(|Q|)).  Every kernel evaluates phi the same way, from the
-// hardware exp2 / reciprocal / log2 (phi_fast, within the sum-product
-// tolerance of -logf(tanhf)), and sums a row's phi values in ascending edge
-// order, so sum-product, too, decodes bit for bit alike on every kernel and
-// tile.  This translation unit is compiled with
+// phi(total - phi(|Q|)).  Every kernel evaluates phi the same way, in log2
+// units from the hardware exp2 / reciprocal / log2 (phi2_fast, within the
+// sum-product tolerance of -logf(tanhf)), and sums a row's phi values in
+// ascending edge order, so sum-product, too, decodes bit for bit alike on
+// every kernel and tile.  This translation unit is compiled with
 // -ffp-contract=off, so every product and sum rounds on its own, as in the
 // reference: min-sum is bit-identical to the CPU restatement, including
 // iteration counts and valid flags.  Early stop checks every row's parity of
@@ -393,12 +393,12 @@ __device__ __forceinline__ void row_update(const App& A, float* __restrict__ Rm,
         for (int i = 0; i < MAXD; ++i)
             if (r.col[i] >= 0) {
                 neg ^= (r.q[i] < 0.0f) ? 1u : 0u;
-                total = total + phi_fast(fabsf(r.q[i]));
+                total = total + phi2_fast(fabsf(r.q[i]) * kLog2eF);
             }
 #pragma unroll
         for (int i = 0; i < MAXD; ++i)
             if (r.col[i] >= 0) {
-                const float mag = phi_fast(total - phi_fast(fabsf(r.q[i])));
+                const float mag = phi2_fast(total - phi2_fast(fabsf(r.q[i]) * kLog2eF)) * kLn2F;
                 const float rn = (neg ^ ((r.q[i] < 0.0f) ? 1u : 0u)) ? -mag : mag;
                 if (UPDATE_APP) A.st(r.col[i], r.q[i] + rn);
                 Rm[r.base + i * r.stride] = rn;
@@ -476,12 +476,12 @@ __device__ __forceinline__ void generic_row(const ck_ldpc_code& c, const App& A,
         for (int e = e0; e < e1; ++e) {
             const float q = A.ld(c.cols[e]) - (first ? 0.0f : Rm[e]);
             neg ^= (q < 0.0f) ? 1u : 0u;
-            total = total + phi_fast(fabsf(q));
+            total = total + phi2_fast(fabsf(q) * kLog2eF);
         }
         for (int e = e0; e < e1; ++e) {
             const int col = c.cols[e];
             const float q = A.ld(col) - (first ? 0.0f : Rm[e]);
-            const float mag = phi_fast(total - phi_fast(fabsf(q)));
+            const float mag = phi2_fast(total - phi2_fast(fabsf(q) * kLog2eF)) * kLn2F;
             const float rn = (neg ^ ((q < 0.0f) ? 1u : 0u)) ? -mag : mag;
             if (UPDATE_APP) A.st(col, q + rn);
             Rm[e] = rn;

@@ -726,17 +726,30 @@ __device__ __forceinline__ CheckState qc_ms_row(const ck_ldpc_code& c, const int
 }
 
 // phi(x) = -ln(tanh(x / 2)) = ln((1 + e^-x) / (1 - e^-x)) for x clipped to
-// [kPhiMin, kPhiMax], from the hardware exp2, reciprocal and log2.  Below
-// 1/16, 1 - e^-x would lose digits to cancellation and comes from its Taylor
-// series (relative error of the four terms below 2e-7 there).  Within about
-// 1e-6 relative of the float64 value where phi is above 1e-2; smaller values
-// (x above ~5) carry an absolute error near 1e-7.
-__device__ __forceinline__ float phi_fast(float x) {
-    x = fminf(fmaxf(x, kPhiMin), kPhiMax);
-    const float t = __builtin_amdgcn_exp2f(x * -1.44269504f);  // e^-x
-    const float series = x * (1.0f - x * (0.5f - x * (0.16666667f - x * 0.041666668f)));
+// [kPhiMin, kPhiMax], taken in log2 units: phi2(y) = phi(y ln 2) / ln 2 =
+// log2((1 + 2^-y) / (1 - 2^-y)), from the hardware exp2, reciprocal and log2
+// with no scaling in or out.  A row keeps its phi values and their total in
+// these units: |Q| is scaled by log2(e) on the way in and the new message
+// by ln 2 on the way out, two multiplies per edge instead of four.  Below
+// x = 1/16, 1 - e^-x would lose digits to cancellation and comes from its
+// Taylor series (relative error of the four terms below 2e-7 there; written
+// with fused multiply-adds, which the file's no-contraction flag would not
+// form by itself).  Within about 1e-6 relative of the float64 value where
+// phi is above 1e-2; smaller values (x above ~5) carry an absolute error
+// near 1e-7.
+constexpr float kLog2eF = 1.44269504f;
+constexpr float kLn2F = 0.69314718f;
+
+__device__ __forceinline__ float phi2_fast(float y) {
+    y = fminf(fmaxf(y, kPhiMin * kLog2eF), kPhiMax * kLog2eF);
+    const float t = __builtin_amdgcn_exp2f(-y);  // e^-x, x = y ln 2
+    const float x = y * kLn2F;
+    const float series =
+        x * __builtin_fmaf(-x, __builtin_fmaf(-x, __builtin_fmaf(-x, 0.041666668f, 0.16666667f),
+                                              0.5f),
+                           1.0f);
     const float den = x < 0.0625f ? series : 1.0f - t;
-    return __builtin_amdgcn_logf((1.0f + t) * __builtin_amdgcn_rcpf(den)) * 0.69314718f;
+    return __builtin_amdgcn_logf((1.0f + t) * __builtin_amdgcn_rcpf(den));
 }
 
 // One sum-product check row over NT slots of the lane (edges e = t*P + p);
@@ -773,7 +786,7 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
         const int e = t * P + p;
         const bool valid = (t < NT - S) || (e < dc);
         q[t] = *addr[t] - rold[t];
-        ph[t] = valid ? phi_fast(fabsf(q[t])) : 0.0f;
+        ph[t] = valid ? phi2_fast(fabsf(q[t]) * kLog2eF) : 0.0f;
         neg ^= (valid && q[t] < 0.0f) ? 1u : 0u;
     }
     if constexpr (P == 1) {
@@ -806,7 +819,7 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
         if ((t < NT - S) || (e < dc)) {
-            const float mag = phi_fast(total - ph[t]);
+            const float mag = phi2_fast(total - ph[t]) * kLn2F;
             const float rn = (neg ^ ((q[t] < 0.0f) ? 1u : 0u)) ? -mag : mag;
             if (UPDATE_APP) *addr[t] = q[t] + rn;
             Rm[static_cast<size_t>(s0 + e) * Z + r] = rn;
