@@ -58,15 +58,20 @@ def _agree(out: Any, ref: Any) -> bool:
     return all(torch.equal(a, b) for a, b in zip(out, ref))
 
 
-def tune(code: Any, method: Any, llrs: Any, *, repeats: int = 3,
+def tune(code: Any, method: Any, llrs: Any, *, rounds: int = 7,
          verbose: bool = False) -> tuple[tuple[int, int, int, int], list[dict]]:
     """Time every candidate tile on ``llrs`` (``(R, n)`` float32, GPU); the
-    fastest tile and the table of timings.  Raises if two tiles disagree."""
+    fastest tile and the table of timings.  Raises if two tiles disagree.
+
+    The tiles are timed in interleaved rounds, each tile once a round in a
+    rotating order, and ranked by their median: a drift of the clock over
+    the run then reaches every tile alike instead of favouring the tiles
+    timed in a fast stretch."""
     import torch
 
     from .coding.ldpc import _decode
 
-    rows = []
+    feasible = []
     ref = None
     for tiles in candidates(code, method):
         try:
@@ -78,21 +83,27 @@ def tune(code: Any, method: Any, llrs: Any, *, repeats: int = 3,
         if ref is None:
             ref = out
         elif not _agree(out, ref):
-            raise RuntimeError(f"tile {tiles} disagrees with {rows[0]['tiles']}")
-        times = []
-        for _ in range(repeats):
+            raise RuntimeError(f"tile {tiles} disagrees with {feasible[0]}")
+        feasible.append(tiles)
+    if not feasible:
+        raise RuntimeError("no quasi-cyclic tile applies to this code")
+    times: dict = {t: [] for t in feasible}
+    for r in range(rounds):
+        for j in range(len(feasible)):
+            tiles = feasible[(j + r) % len(feasible)]
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record()
             _decode(code, llrs, method, tiles=tiles)
             b.record()
             b.synchronize()
-            times.append(a.elapsed_time(b))
-        rows.append({"tiles": list(tiles), "ms": min(times)})
+            times[tiles].append(a.elapsed_time(b))
+    rows = []
+    for tiles in feasible:
+        t = sorted(times[tiles])
+        rows.append({"tiles": list(tiles), "ms": t[len(t) // 2]})
         if verbose:
-            print(f"  {tiles}: {min(times):.4f} ms", flush=True)
-    if not rows:
-        raise RuntimeError("no quasi-cyclic tile applies to this code")
+            print(f"  {tiles}: median {t[len(t) // 2]:.4f} ms (min {t[0]:.4f})", flush=True)
     best = min(rows, key=lambda r: r["ms"])
     return tuple(best["tiles"]), rows  # type: ignore[return-value]
 
