@@ -115,6 +115,10 @@ def _parse() -> argparse.Namespace:
                     help="serving: offered load (fraction of saturated) the p99 bound is stated at")
     ap.add_argument("--p99-bound-ms", type=float, default=10.0,
                     help="serving: stated p99 frame-latency bound at --bound-load")
+    ap.add_argument("--demap-per-turn", action="store_true",
+                    help="serving: the requests admitted in a loop turn demapped together "
+                         "(SegmentsDemapper, up to 16 requests a launch) instead of one "
+                         "demapper launch per request")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device twice (counter calibration) and exit")
     return ap.parse_args()
@@ -945,7 +949,7 @@ def shaped_served(args, code, c, fr, nv: float, F: int, method, dev) -> dict:
     frames closes the run (its time is inside the measured wall clock)."""
     import torch
 
-    from commkit_amd.mapping.demap import llr_permuted
+    from commkit_amd.mapping.demap import SegmentsDemapper, llr_permuted
     from commkit_amd.serve import Schedule, Server, drive
 
     rng = np.random.Generator(np.random.PCG64(11))
@@ -961,9 +965,16 @@ def shaped_served(args, code, c, fr, nv: float, F: int, method, dev) -> dict:
     rx = fr.rx.reshape(-1)
     perm32 = fr.perm.to(device=dev, dtype=torch.int32).contiguous()
 
+    seg_dm = SegmentsDemapper(c, "exact", dev) if args.demap_per_turn else None
+
     def fill(slot, i):
         a, f = int(first[i]), int(sizes[i])
-        llr_permuted(c, rx[a * S:(a + f) * S], nv, "exact", perm32, f, out=slot.llrs)
+        if seg_dm is not None:
+            seg_dm.add(rx[a * S:(a + f) * S], slot.llrs, noise_var=nv, perm=perm32)
+        else:
+            llr_permuted(c, rx[a * S:(a + f) * S], nv, "exact", perm32, f, out=slot.llrs)
+
+    flush = seg_dm.flush if seg_dm is not None else None
 
     bits_out = torch.empty((F, code.k), dtype=torch.int8, device=dev)
     iters_out = torch.empty(F, dtype=torch.int32, device=dev)
@@ -976,9 +987,9 @@ def shaped_served(args, code, c, fr, nv: float, F: int, method, dev) -> dict:
     srv = Server([code], method, hbm_budget=16 << 30, max_inflight=args.max_inflight, device=dev,
                  max_launch_frames=args.max_launch_frames or None)  # fmt: skip
     warm = Schedule(np.zeros(4), np.arange(4), np.zeros(4, np.int64), sizes[:4], np.zeros(4, np.int64))
-    drive(srv, warm, fill)
+    drive(srv, warm, fill, before_step=flush)
     t0 = time.perf_counter()
-    res = drive(srv, sched, fill, on_done=on_done)
+    res = drive(srv, sched, fill, on_done=on_done, before_step=flush)
     ccdm = fr.pas.matcher
     payload, dm_ok = ccdm._dematch_labels(bits_out[:, : ccdm.n])
     payload = torch.cat([payload, bits_out[:, ccdm.n:]], dim=1)
@@ -1005,7 +1016,7 @@ def shaped_served(args, code, c, fr, nv: float, F: int, method, dev) -> dict:
         "mean_iterations": it_sum / F,
         "tail_hbm_frac": round(tail_bytes / res.elapsed_s / 1e9 / HBM_PEAK_GBS, 4),
         "note": "gbps: the code's information bits over the wall clock of the loop (host "
-                "admission, per-request demap into the slot, batched decode); the payload "
+                "admission, each request demapped into its slot, batched decode); the payload "
                 "rate adds the dematcher launch after the loop",
     }
 
@@ -1030,7 +1041,7 @@ def run_serving(args, dev) -> tuple[dict, bool]:
 
     from commkit_amd import Constellation, MinSum
     from commkit_amd.coding.ldpc import _decode
-    from commkit_amd.mapping.demap import RowsDemapper, llr_rows
+    from commkit_amd.mapping.demap import RowsDemapper, SegmentsDemapper, llr_rows
     from commkit_amd.operating import waterfall_db
     from commkit_amd.serve import Schedule, Server, burst_schedule, drive, poisson_schedule
     from commkit_amd.workloads import MIXED_CODES, MIXED_ORDERS, make_frames, seeded_code
@@ -1058,14 +1069,29 @@ def run_serving(args, dev) -> tuple[dict, bool]:
                  max_launch_frames=args.max_launch_frames or None)  # fmt: skip
     mean_bits = float(np.mean([codes[ci].k for ci, _ in pairs])) * (1 + max_frames) / 2
 
-    demappers = {o: RowsDemapper(consts[o], "maxlog", dev) for o in MIXED_ORDERS}
+    # Each admitted request is demapped into its ring slot by a launch of
+    # its own, or with --demap-per-turn queued per constellation and
+    # launched once per loop turn (one launch per 16 requests).
+    if args.demap_per_turn:
+        rows_dm = {}
+        seg_dm = {o: SegmentsDemapper(consts[o], "maxlog", dev) for o in MIXED_ORDERS}
+    else:
+        rows_dm = {o: RowsDemapper(consts[o], "maxlog", dev) for o in MIXED_ORDERS}
+        seg_dm = {}
+
+    def flush_all():
+        for d in seg_dm.values():
+            d.flush()
 
     def make_fill(sched: Schedule):
         def fill(slot, i):
             ci, order = pairs[int(sched.stream[i])]
             rx, inv, _ = pools[(ci, order)][int(sched.snr_bin[i])]
             f = slot.frames
-            demappers[order](rx[:f], inv[:f], slot.llrs)
+            if seg_dm:
+                seg_dm[order].add(rx[:f], slot.llrs, inv_s2=inv[:f])
+            else:
+                rows_dm[order](rx[:f], inv[:f], slot.llrs)
         return fill
 
     def codes_of(sched: Schedule) -> Schedule:
@@ -1077,7 +1103,7 @@ def run_serving(args, dev) -> tuple[dict, bool]:
     n_sat = 3000
     warm = codes_of(burst_schedule(64, streams=streams, num_codes=3, max_frames=max_frames,
                                    snr_bins=bins, seed=2))  # fmt: skip
-    drive(srv, warm, make_fill(warm))
+    drive(srv, warm, make_fill(warm), before_step=flush_all)
     burst = codes_of(burst_schedule(n_sat, streams=streams, num_codes=3, max_frames=max_frames,
                                     snr_bins=bins, seed=1))  # fmt: skip
     kept: dict = {}
@@ -1087,7 +1113,7 @@ def run_serving(args, dev) -> tuple[dict, bool]:
             kept[i] = (slot.llrs.clone(), slot.bits.clone(), slot.app.clone(),
                        slot.valid.clone(), slot.iterations.clone())  # fmt: skip
 
-    sat = drive(srv, burst, make_fill(burst), on_done=keep)
+    sat = drive(srv, burst, make_fill(burst), on_done=keep, before_step=flush_all)
     # Algorithmic bytes of the saturated run: a frame's iteration count is
     # the same alone and in any batch, so pool frames are decoded per call,
     # outside the timed region, for the counts.
@@ -1117,7 +1143,7 @@ def run_serving(args, dev) -> tuple[dict, bool]:
         rate = frac * sat.gbps * 1e9 / mean_bits
         sched = codes_of(poisson_schedule(rate, args.serve_seconds, streams=streams, num_codes=3,
                                           max_frames=max_frames, seed=int(frac * 1000)))  # fmt: skip
-        res = drive(srv, sched, make_fill(sched))
+        res = drive(srv, sched, make_fill(sched), before_step=flush_all)
         p99 = round(res.percentile(99), 3)
         loads.append({
             "offered_fraction": frac,

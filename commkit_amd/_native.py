@@ -34,6 +34,7 @@ SYMBOLS: dict[str, tuple[Any, list[Any]]] = {
     "ck_llr_separable_maxlog": (_i32, [_p, _i64, _p, _p, _f64, _p, _p]),
     "ck_llr_separable": (_i32, [_p, _i64, _p, _p, _f64, _i32, _p, _p]),
     "ck_llr_separable_rows": (_i32, [_p, _i64, _i64, _p, _p, _p, _i32, _i32, _p, _p]),
+    "ck_llr_separable_segments": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p]),
     "ck_llr_points": (_i32, [_p, _i64, _p, _p, _p, _i32, _i32, _f64, _i32, _p, _p]),
     "ck_llr_separable_permuted": (_i32, [_p, _i64, _p, _p, _f64, _i32, _i32, _p, _p, _i64, _i64,
                                          _p]),

@@ -596,6 +596,26 @@ __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float
 #define CK_SEPARABLE_ARGS \
     x, n, re, im, inv_s2f, inv_s2, exact, out, inv_rows, row_len, perm, period, limit
 
+// The k LLRs of one symbol v into row: both axes of the staged tables.
+template <int KB, bool CLOSED>
+__device__ __forceinline__ void separable_symbol(AxisTab<KB>* tab, float2 v, float inv_s2f,
+                                                 double inv_s2, float inv_step0, float inv_step1,
+                                                 int exact, float* row) {
+    if (CLOSED) {
+        axis_closed<KB>(tab[0], v.x, inv_s2, inv_step0, row);
+        axis_closed<KB>(tab[1], v.y, inv_s2, inv_step1, row);
+    } else {
+        // The table index is opaque to the compiler, so the level tables are
+        // read from LDS per symbol (broadcast reads) instead of held in
+        // registers across the caller's loop: 2 x 2^KB float64 values per
+        // axis would spill.
+        int z = 0;
+        asm volatile("" : "+v"(z));
+        axis_levels<KB>(tab[z], v.x, inv_s2f, inv_s2, inv_step0, exact != 0, row);
+        axis_levels<KB>(tab[z + 1], v.y, inv_s2f, inv_s2, inv_step1, exact != 0, row);
+    }
+}
+
 template <int KB, bool CLOSED>
 __device__ __forceinline__ void separable_body(CK_SEPARABLE_PARAMS) {
     constexpr int K = 2 * KB;
@@ -620,19 +640,8 @@ __device__ __forceinline__ void separable_body(CK_SEPARABLE_PARAMS) {
                 inv_s2 = inv_rows[r];
                 inv_s2f = static_cast<float>(inv_s2);
             }
-            if (CLOSED) {
-                axis_closed<KB>(tab[0], v.x, inv_s2, inv_step0, row);
-                axis_closed<KB>(tab[1], v.y, inv_s2, inv_step1, row);
-            } else {
-                // The table index is opaque to the compiler, so the level
-                // tables are read from LDS per symbol (broadcast reads)
-                // instead of held in registers across the loop: 2 x 2^KB
-                // float64 values per axis would spill.
-                int z = 0;
-                asm volatile("" : "+v"(z));
-                axis_levels<KB>(tab[z], v.x, inv_s2f, inv_s2, inv_step0, exact != 0, row);
-                axis_levels<KB>(tab[z + 1], v.y, inv_s2f, inv_s2, inv_step1, exact != 0, row);
-            }
+            separable_symbol<KB, CLOSED>(tab, v, inv_s2f, inv_s2, inv_step0, inv_step1, exact,
+                                         row);
         }
         __syncthreads();
         const long long left = n - base;
@@ -658,6 +667,85 @@ template <int KB, bool CLOSED>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void
 llr_separable_kernel_w4(CK_SEPARABLE_PARAMS) {
     separable_body<KB, CLOSED>(CK_SEPARABLE_ARGS);
+}
+
+// Up to kMaxSegments requests of one constellation in one launch, passed by
+// value: segment j demaps its symbols x[j][0 .. n[j]) into out[j] (the plain
+// layout, or the bit layout of perm[j] with period and limit as
+// ck_llr_separable_permuted), 1/sigma^2 inv_s2[j] or, when inv_rows[j] is
+// set, inv_rows[j][i / row_len[j]].  tile_end[j] is the prefix sum of the
+// segments' 256-symbol tiles: a workgroup takes whole tiles of one segment,
+// found by a uniform scan.
+constexpr int kMaxSegments = 16;
+
+struct Segments {
+    const float2* x[kMaxSegments];
+    float* out[kMaxSegments];
+    const double* inv_rows[kMaxSegments];
+    const int* perm[kMaxSegments];
+    long long n[kMaxSegments];
+    long long row_len[kMaxSegments];
+    long long period[kMaxSegments];
+    long long limit[kMaxSegments];
+    long long tile_end[kMaxSegments];
+    double inv_s2[kMaxSegments];
+    int count;
+};
+
+template <int KB, bool CLOSED>
+__device__ __forceinline__ void segments_body(const Segments& sg, const ck_axis& re,
+                                              const ck_axis& im, int exact) {
+    constexpr int K = 2 * KB;
+    __shared__ AxisTab<KB> tab[2];
+    __shared__ float tile[kThreads * (K | 1)];
+    stage_axis<KB>(tab[0], re, CLOSED);
+    stage_axis<KB>(tab[1], im, CLOSED);
+    __syncthreads();
+    const float inv_step0 = 1.0f / (tab[0].levf[1] - tab[0].levf[0]);
+    const float inv_step1 = 1.0f / (tab[1].levf[1] - tab[1].levf[0]);
+    float* row = tile + threadIdx.x * tile_stride(K);
+    const long long tiles = sg.tile_end[sg.count - 1];
+    for (long long tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
+        int j = 0;
+        while (j + 1 < sg.count && sg.tile_end[j] <= tl) ++j;
+        const long long base = (tl - (j > 0 ? sg.tile_end[j - 1] : 0)) * kThreads;
+        const long long n = sg.n[j];
+        const long long i = base + threadIdx.x;
+        if (i < n) {
+            const float2 v = sg.x[j][i];
+            double inv_s2 = sg.inv_s2[j];
+            const double* rows = sg.inv_rows[j];
+            if (rows != nullptr) {
+                const long long len = sg.row_len[j];
+                inv_s2 = rows[(n <= 0xffffffffLL)
+                                  ? static_cast<long long>(static_cast<unsigned>(i) /
+                                                           static_cast<unsigned>(len))
+                                  : i / len];
+            }
+            separable_symbol<KB, CLOSED>(tab, v, static_cast<float>(inv_s2), inv_s2, inv_step0,
+                                         inv_step1, exact, row);
+        }
+        __syncthreads();
+        const int count = n - base < kThreads ? static_cast<int>(n - base) : kThreads;
+        if (sg.perm[j] != nullptr)
+            store_tile_perm<K>(tile, sg.out[j], base, count, sg.perm[j], sg.period[j],
+                               sg.limit[j]);
+        else
+            store_tile(tile, sg.out[j], base, count, K);
+        __syncthreads();
+    }
+}
+
+template <int KB, bool CLOSED>
+__global__ __launch_bounds__(kThreads) void llr_segments_kernel(Segments sg, ck_axis re,
+                                                                ck_axis im, int exact) {
+    segments_body<KB, CLOSED>(sg, re, im, exact);
+}
+
+template <int KB, bool CLOSED>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void
+llr_segments_kernel_w4(Segments sg, ck_axis re, ck_axis im, int exact) {
+    segments_body<KB, CLOSED>(sg, re, im, exact);
 }
 
 // m_a - m_c of two points in float64, factored per component.
@@ -943,6 +1031,104 @@ extern "C" int32_t ck_llr_separable_permuted(const void* x, int64_t n, const ck_
                                            period, limit)
                   : launch_separable<false>(x, m, re, im, inv_s2, exact, out, stream, nullptr, 1,
                                             perm, period, limit);
+}
+
+namespace {
+
+template <int KB, bool CLOSED>
+constexpr auto segments_kernel() {
+    if constexpr (!CLOSED && KB <= 4)
+        return llr_segments_kernel_w4<KB, CLOSED>;
+    else
+        return llr_segments_kernel<KB, CLOSED>;
+}
+
+template <bool CLOSED>
+int32_t launch_segments(const Segments& sg, const ck_axis* re, const ck_axis* im, int exact,
+                        hipStream_t s) {
+    const long long tiles = sg.tile_end[sg.count - 1];
+    const int grid = grid_for(tiles * kThreads);
+    switch (re->num_bits) {
+#define CK_CASE(KB)                                                                        \
+    case KB:                                                                               \
+        hipLaunchKernelGGL((segments_kernel<KB, CLOSED>()), dim3(grid), dim3(kThreads), 0, \
+                           s, sg, *re, *im, exact);                                        \
+        break;
+        CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
+#undef CK_CASE
+    }
+    return ck::launched("llr_separable_segments");
+}
+
+}  // namespace
+
+extern "C" int32_t ck_llr_separable_segments(const ck_llr_segment* segs, int32_t count,
+                                             const ck_axis* re, const ck_axis* im,
+                                             int32_t closed, int32_t exact, void* stream) {
+    CK_REQUIRE(count >= 0, "count = %d < 0", count);
+    CK_REQUIRE(closed == 0 || closed == 1, "closed must be 0 or 1, got %d", closed);
+    CK_REQUIRE(exact == 0 || exact == 1, "exact must be 0 or 1, got %d", exact);
+    CK_REQUIRE(!(closed && exact), "the closed form is max-log only");
+    if (count == 0) return CK_OK;
+    CK_REQUIRE(segs != nullptr, "NULL segments");
+    if (int32_t st = check_axis(re, "re", closed != 0)) return st;
+    if (int32_t st = check_axis(im, "im", closed != 0)) return st;
+    CK_REQUIRE(re->num_bits == im->num_bits,
+               "separable axes must carry the same number of bits (%d, %d)", re->num_bits,
+               im->num_bits);
+    const int64_t k = 2 * re->num_bits;
+    // Every segment is checked before anything is launched.
+    for (int32_t j = 0; j < count; ++j) {
+        const ck_llr_segment& g = segs[j];
+        CK_REQUIRE(g.n >= 0, "segment %d: n = %lld < 0", j, static_cast<long long>(g.n));
+        if (g.inv_rows != nullptr)
+            CK_REQUIRE(g.row_len >= 1, "segment %d: row_len %lld < 1", j,
+                       static_cast<long long>(g.row_len));
+        else
+            CK_REQUIRE(g.inv_s2 > 0.0 && g.inv_s2 < 1e300,
+                       "segment %d: inv_s2 must be positive and finite", j);
+        if (g.perm != nullptr) {
+            CK_REQUIRE(g.period >= 1 && g.limit >= 0 && g.limit % g.period == 0,
+                       "segment %d: limit %lld is not a whole number of blocks of %lld", j,
+                       static_cast<long long>(g.limit), static_cast<long long>(g.period));
+            CK_REQUIRE(g.limit <= g.n * k, "segment %d: limit %lld past the %lld LLRs", j,
+                       static_cast<long long>(g.limit), static_cast<long long>(g.n * k));
+        }
+        CK_REQUIRE(g.n == 0 || (g.x != nullptr && g.out != nullptr),
+                   "segment %d: NULL symbols or output", j);
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Segments sg{};
+    long long tiles = 0;
+    for (int32_t j = 0; j <= count; ++j) {
+        if (j == count || sg.count == kMaxSegments) {
+            if (sg.count > 0) {
+                const int32_t st = closed ? launch_segments<true>(sg, re, im, 0, s)
+                                          : launch_segments<false>(sg, re, im, exact, s);
+                if (st != CK_OK) return st;
+            }
+            sg = Segments{};
+            tiles = 0;
+            if (j == count) break;
+        }
+        const ck_llr_segment& g = segs[j];
+        // With a layout, only the symbols holding LLRs below the limit.
+        const long long n = g.perm != nullptr ? (g.limit + k - 1) / k : g.n;
+        if (n == 0) continue;
+        const int c = sg.count++;
+        sg.x[c] = static_cast<const float2*>(g.x);
+        sg.out[c] = g.out;
+        sg.inv_rows[c] = g.inv_rows;
+        sg.perm[c] = g.perm;
+        sg.n[c] = n;
+        sg.row_len[c] = g.inv_rows != nullptr ? g.row_len : 1;
+        sg.period[c] = g.perm != nullptr ? g.period : 1;
+        sg.limit[c] = g.perm != nullptr ? g.limit : 0;
+        sg.inv_s2[c] = g.inv_rows != nullptr ? 1.0 : g.inv_s2;
+        tiles += (n + kThreads - 1) / kThreads;
+        sg.tile_end[c] = tiles;
+    }
+    return CK_OK;
 }
 
 extern "C" int32_t ck_llr_points(const void* x, int64_t n, const void* points,

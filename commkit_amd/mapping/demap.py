@@ -329,6 +329,8 @@ class RowsDemapper:
         if c._axes is None:
             raise ValueError("RowsDemapper: the constellation is not separable per axis")
         self.device = torch.device(device)
+        if self.device.index is None:  # "cuda": the current device
+            self.device = torch.device("cuda", torch.cuda.current_device())
         with torch.cuda.device(self.device):
             self._re, self._im, self._keep = axis_tables(c, self.device)
             self._stream = _native.stream_of(self.device)
@@ -353,3 +355,122 @@ class RowsDemapper:
                 x.data_ptr(), R, S, ctypes.byref(self._re), ctypes.byref(self._im),
                 inv_s2.data_ptr(), self._closed, self._exact, out.data_ptr(), self._stream))  # fmt: skip
         return out
+
+
+class CkLlrSegment(ctypes.Structure):
+    """``ck_llr_segment`` of ``include/ck_tail.h``: one request of
+    ``ck_llr_separable_segments``."""
+
+    _fields_ = [
+        ("x", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("n", ctypes.c_int64),
+        ("inv_s2", ctypes.c_double),
+        ("inv_rows", ctypes.c_void_p),
+        ("row_len", ctypes.c_int64),
+        ("perm", ctypes.c_void_p),
+        ("period", ctypes.c_int64),
+        ("limit", ctypes.c_int64),
+    ]
+
+
+class SegmentsDemapper:
+    """Many requests of one separable constellation demapped in one launch
+    (per 16 requests), bound once like :class:`RowsDemapper`.
+
+    For request loops: each request is queued with :meth:`add` (its
+    symbols, its output and its noise: a scalar ``noise_var`` or one
+    ``1 / sigma^2`` per row; optionally a code's bit layout ``perm`` as
+    :func:`llr_permuted`), and :meth:`flush` launches every queued request
+    on the stream current at construction (a 16th queued request launches
+    the 16 at once, so a long queue is demapped while it is still being
+    filled).  Each request's LLRs equal its
+    own :func:`llr_any` / :class:`RowsDemapper` / :func:`llr_permuted` call
+    bit for bit.  The demapper holds the queued tensors until :meth:`flush`
+    has enqueued the launch; after that, memory the caller frees is reused
+    only by work ordered after the launch on the same stream (torch's
+    caching allocator), so the tensors need not outlive the call.
+    """
+
+    BATCH = 16  # requests one launch takes (ck_llr_separable_segments)
+
+    def __init__(self, c: "Constellation", method: str, device: Any) -> None:
+        import torch
+
+        check_method(method)
+        if c._axes is None:
+            raise ValueError("SegmentsDemapper: the constellation is not separable per axis")
+        self.device = torch.device(device)
+        if self.device.index is None:  # "cuda": the current device
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        with torch.cuda.device(self.device):
+            self._re, self._im, self._keep = axis_tables(c, self.device)
+            self._stream = _native.stream_of(self.device)
+        self._closed = int(closed_form(c._axes, method))
+        self._exact = int(method == "exact")
+        self.k = c.bits_per_symbol
+        self._lib = _native.lib()
+        self._queue: list[CkLlrSegment] = []
+        self._held: list[Any] = []  # the queued requests' tensors
+
+    def add(self, x: Any, out: Any, *, noise_var: float | None = None, inv_s2: Any = None,
+            perm: Any = None) -> None:
+        """Queue the LLRs of ``x`` (complex64, contiguous, ``(R, S)`` with
+        ``inv_s2`` of ``(R,)`` float64, else any shape) into ``out``
+        (float32, contiguous: ``x.numel() * k`` values, or with ``perm`` of
+        ``P`` entries ``(blocks, P)`` holding the first ``blocks * P``
+        LLRs in the code's bit layout)."""
+        import torch
+
+        if x.dtype != torch.complex64 or not x.is_contiguous() or x.device != self.device:
+            raise ValueError("SegmentsDemapper.add: x must be contiguous complex64 on the "
+                             "demapper's device")  # fmt: skip
+        if out.dtype != torch.float32 or not out.is_contiguous() or out.device != self.device:
+            raise ValueError("SegmentsDemapper.add: out must be contiguous float32 on the "
+                             "demapper's device")  # fmt: skip
+        n = int(x.numel())
+        seg = CkLlrSegment(x.data_ptr(), out.data_ptr(), n, 1.0, None, 1, None, 1, 0)
+        if (noise_var is None) == (inv_s2 is None):
+            raise ValueError("SegmentsDemapper.add: give exactly one of noise_var and inv_s2")
+        if inv_s2 is not None:
+            R = int(x.shape[0]) if x.dim() == 2 else -1
+            if (R < 1 or inv_s2.dtype != torch.float64 or inv_s2.numel() != R
+                    or not inv_s2.is_contiguous() or inv_s2.device != self.device):
+                raise ValueError("SegmentsDemapper.add: inv_s2 must be (R,) contiguous float64 "
+                                 "for x of (R, S)")  # fmt: skip
+            seg.inv_rows, seg.row_len = inv_s2.data_ptr(), int(x.shape[1])
+        else:
+            seg.inv_s2 = 1.0 / max(float(noise_var), 1e-20)
+        if perm is not None:
+            if (perm.dtype != torch.int32 or not perm.is_contiguous()
+                    or perm.device != self.device or out.dim() != 2
+                    or int(out.shape[1]) != perm.numel()):
+                raise ValueError("SegmentsDemapper.add: perm must be contiguous int32 of P "
+                                 "entries and out (blocks, P)")  # fmt: skip
+            seg.perm, seg.period = perm.data_ptr(), int(perm.numel())
+            seg.limit = int(out.numel())
+            if seg.limit > n * self.k:
+                raise ValueError(f"SegmentsDemapper.add: {out.shape[0]} blocks need "
+                                 f"{-(-seg.limit // self.k)} symbols, got {n}")  # fmt: skip
+        elif out.numel() != n * self.k:
+            raise ValueError(f"SegmentsDemapper.add: out must hold {n * self.k} LLRs")
+        self._queue.append(seg)
+        self._held.append((x, out, inv_s2, perm))
+        if len(self._queue) == self.BATCH:  # a full launch: start it now
+            self.flush()
+
+    def __len__(self) -> int:
+        return len(self._queue)
+
+    def flush(self) -> None:
+        """Launch every queued request (one launch per 16)."""
+        if not self._queue:
+            return
+        arr = (CkLlrSegment * len(self._queue))(*self._queue)
+        self._queue = []
+        try:
+            _native.check(self._lib.ck_llr_separable_segments(
+                arr, len(arr), ctypes.byref(self._re), ctypes.byref(self._im), self._closed,
+                self._exact, self._stream))  # fmt: skip
+        finally:
+            self._held = []

@@ -348,7 +348,8 @@ class LoadResult:
 def drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None], *,
           clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] | None = None,
           idle_s: float = 0.0, on_done: Callable[[Slot, int], None] | None = None,
-          admit_slice_s: float = 1e-3) -> LoadResult:
+          admit_slice_s: float = 1e-3,
+          before_step: Callable[[], None] | None = None) -> LoadResult:
     """Run ``schedule`` against ``server``: each request is admitted when it
     has arrived (or, when the budget is full, as soon as room frees up), its
     LLRs written by ``fill(slot, request index)``, submitted, and launched by
@@ -357,8 +358,12 @@ def drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None],
     the server's stream reads the results before the slot can be reused) and
     released.  A loop turn admits for at most ``admit_slice_s`` seconds, so a
     long backlog is launched in pieces while the rest is still being admitted
-    instead of leaving the device idle.  Returns per-frame latencies
-    (completion seen minus arrival)."""
+    instead of leaving the device idle.  ``before_step()``, if given, runs
+    after each turn's fills and before its :meth:`Server.step` (a fill that
+    only queues its work, as :class:`~commkit_amd.mapping.demap.
+    SegmentsDemapper.add` does, launches it there: the step's launch waits
+    for the work enqueued on the caller's stream).  Returns per-frame
+    latencies (completion seen minus arrival)."""
     n = len(schedule)
     t_arr = schedule.t
     t_done = np.full(n, np.nan)
@@ -390,6 +395,8 @@ def drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None],
             server.submit(i)
             k += 1
         backlog = backlog[k:]
+        if before_step is not None:
+            before_step()
         if server.step() > 0:
             launches += 1
         for tag in server.poll():

@@ -110,6 +110,36 @@ int32_t ck_llr_separable_permuted(const void* x, int64_t n, const ck_axis* re,
                                   int32_t exact, float* out, const int32_t* perm,
                                   int64_t period, int64_t limit, void* stream);
 
+/* One request of ck_llr_separable_segments: the LLRs of x[0 .. n) into
+ * out, as ck_llr_separable (perm NULL: (n, k) float32) or as
+ * ck_llr_separable_permuted (perm, period, limit: the code's bit layout;
+ * only the symbols holding LLRs below limit are demapped), with
+ * 1/sigma^2 = inv_s2 or, when inv_rows is set, inv_rows[i / row_len]
+ * (float64, device).  All pointers are device pointers. */
+typedef struct ck_llr_segment {
+    const void* x;           /* (n,) complex64                              */
+    float* out;              /* LLRs                                        */
+    int64_t n;               /* symbols                                     */
+    double inv_s2;           /* 1 / sigma^2 when inv_rows is NULL           */
+    const double* inv_rows;  /* (n / row_len,) 1 / sigma^2 per row, or NULL */
+    int64_t row_len;         /* symbols per row (with inv_rows)             */
+    const int32_t* perm;     /* (period,) bit layout, or NULL               */
+    int64_t period;
+    int64_t limit;
+} ck_llr_segment;
+
+/* The requests segs[0 .. count) of one separable constellation demapped in
+ * one launch per 16 of them (closed: the closed form, max-log, uniform;
+ * else level metrics, exact or max-log), each exactly as its own call of
+ * ck_llr_separable / ck_llr_separable_rows / ck_llr_separable_permuted would
+ * give.  segs is a host array, read before the call returns.  No
+ * counterpart in the reference: for request loops (a serving loop demapping
+ * each admitted request into its ring slot) that would otherwise launch
+ * once per request. */
+int32_t ck_llr_separable_segments(const ck_llr_segment* segs, int32_t count,
+                                  const ck_axis* re, const ck_axis* im, int32_t closed,
+                                  int32_t exact, void* stream);
+
 /* LLRs from the metrics of all M points (any constellation).
  * points: (M,) complex128 device; log_prior: (M,) float64 device (zeros =
  * uniform); labels: (M,) int32 packed MSB-first labels; 2 <= M <= 65536,

@@ -10,7 +10,7 @@ import pytest
 
 from commkit_amd import _native
 from commkit_amd.coding.ldpc import CkLdpcCode, CkLdpcMethod
-from commkit_amd.mapping.demap import CkAxis
+from commkit_amd.mapping.demap import CkAxis, CkLlrSegment
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
                       "ck_tail.h")
@@ -48,6 +48,10 @@ def test_struct_layouts_match_header():
     body = re.search(r"typedef struct ck_ldpc_code \{(.*?)\} ck_ldpc_code;", text, re.S).group(1)
     fields = re.findall(r"(\w+)\s*;", body)
     assert [f[0] for f in CkLdpcCode._fields_] == fields
+    body = re.search(r"typedef struct ck_llr_segment \{(.*?)\} ck_llr_segment;", text,
+                     re.S).group(1)  # fmt: skip
+    assert [f[0] for f in CkLlrSegment._fields_] == re.findall(r"(\w+)\s*;", body)
+    assert ctypes.sizeof(CkLlrSegment) == 9 * 8
 
 
 def test_argument_errors_return_status_and_message():
@@ -64,6 +68,15 @@ def test_argument_errors_return_status_and_message():
                             None, None)  # fmt: skip
     assert st == 1 and b"num_checks" in lib.ck_last_error()
     assert lib.ck_ldpc_workspace_bytes(ctypes.byref(CkLdpcCode()), None, 1) == -1
+    # segments are all checked before anything is launched
+    segs = (CkLlrSegment * 2)(CkLlrSegment(1, 1, 10, 1.0, None, 1, None, 1, 0),
+                              CkLlrSegment(1, 1, 10, 1.0, None, 1, 1, 7, 20))
+    st = lib.ck_llr_separable_segments(segs, 2, ctypes.byref(ax), ctypes.byref(ax), 0, 0, None)
+    assert st == 1 and b"segment 1: limit 20 is not a whole number of blocks of 7" in \
+        lib.ck_last_error()
+    segs[1] = CkLlrSegment(1, 1, 10, 0.0, None, 1, None, 1, 0)
+    st = lib.ck_llr_separable_segments(segs, 2, ctypes.byref(ax), ctypes.byref(ax), 0, 0, None)
+    assert st == 1 and b"segment 1: inv_s2 must be positive" in lib.ck_last_error()
 
 
 def test_missing_library_raises(monkeypatch, tmp_path):

@@ -359,3 +359,60 @@ def test_prior_steeper_than_the_likelihood(gpu):
     assert np.isfinite(got).all()
     ok, _ = llr_tolerance_ok(got, ref, floor=FLOOR["exact"])
     assert ok, f"worst error {np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 2**-5)):.3e}"
+
+
+@pytest.mark.parametrize("make,method", [
+    (lambda: Constellation.qam(64), "maxlog"),
+    (lambda: Constellation.qam(16), "exact"),
+    (lambda: Constellation.qam(256).shaped(nu=0.05), "exact"),
+], ids=["64qam-maxlog", "16qam-exact", "256qam-shaped-exact"])  # fmt: skip
+def test_segments_equal_per_request_calls(gpu, make, method):
+    """Twenty requests through a SegmentsDemapper (the 16th launches the
+    first 16, flush the rest): scalar noise, per-row noise and a code's bit
+    layout, sizes from
+    nothing to several tiles; each request's LLRs equal its own call bit for
+    bit, and nothing outside the requests' outputs is written."""
+    import torch
+
+    from commkit_amd.mapping.demap import RowsDemapper, SegmentsDemapper, llr_any, llr_permuted
+
+    c = make()
+    k = c.bits_per_symbol
+    rng = np.random.default_rng(21)
+    dm = SegmentsDemapper(c, method, "cuda")
+    rows_dm = RowsDemapper(c, method, "cuda")
+    period = 36 * k // 2  # blocks end inside a symbol when k is odd per block
+    perm = torch.from_numpy(rng.permutation(period).astype(np.int32)).cuda()
+    iperm = torch.argsort(perm.long())
+    checks = []
+    for j in range(20):
+        kind = j % 3
+        if kind == 0:
+            n = [0, 1, 255, 256, 257, 1000, 3001][j % 7]
+            x = torch.from_numpy(_noisy(c, n, 0.03, seed=j)).cuda()
+            out = torch.full((n, k), float("nan"), device="cuda")
+            dm.add(x, out, noise_var=0.03 + 0.001 * j)
+            checks.append((out, llr_any(c, x, 0.03 + 0.001 * j, method).reshape(n, k)))
+        elif kind == 1:
+            R, S = 1 + j % 4, 97 + 13 * j
+            x = torch.from_numpy(_noisy(c, R * S, 0.05, seed=j).reshape(R, S)).cuda()
+            inv = torch.from_numpy(1.0 / rng.uniform(0.01, 0.2, size=R)).cuda()
+            out = torch.full((R, S * k), float("nan"), device="cuda")
+            dm.add(x, out, inv_s2=inv)
+            checks.append((out, rows_dm(x, inv, torch.empty_like(out))))
+        else:
+            blocks = 1 + j % 5
+            x = torch.from_numpy(_noisy(c, -(-blocks * period // k) + 3, 0.02, seed=j)).cuda()
+            out = torch.full((blocks, period), float("nan"), device="cuda")
+            dm.add(x, out, noise_var=0.02, perm=perm)
+            want = llr_permuted(c, x, 0.02, method, perm, blocks)
+            checks.append((out, want))
+            flat = llr_any(c, x, 0.02, method)[: blocks * period].view(blocks, period)[:, iperm]
+            assert torch.equal(want, flat)
+    assert len(dm) == 4  # the 16th request launched the first 16
+    dm.flush()
+    assert len(dm) == 0
+    for j, (got, want) in enumerate(checks):
+        assert torch.equal(got, want), f"request {j}"
+    with pytest.raises(ValueError, match="exactly one"):
+        dm.add(x, out, perm=perm)

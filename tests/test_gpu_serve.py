@@ -16,9 +16,10 @@ from commkit_amd.workloads import seeded_code
 pytestmark = pytest.mark.gpu
 
 
-def _serve(srv, reqs, fill, check, per_turn=3):
+def _serve(srv, reqs, fill, check, per_turn=3, before_step=None):
     """Admit up to ``per_turn`` requests a turn (as room allows), step, poll
-    and check every completed request."""
+    and check every completed request; ``before_step()`` runs after a turn's
+    fills, before its step."""
     pending = list(reqs)
     live = {}
     checked = 0
@@ -37,6 +38,8 @@ def _serve(srv, reqs, fill, check, per_turn=3):
             srv.submit(req[0])
             live[req[0]] = req
             pending.pop(0)
+        if before_step is not None:
+            before_step()
         srv.step()
         for tag in srv.poll():
             check(srv.slot(tag), live.pop(tag))
@@ -76,14 +79,16 @@ def test_requests_identical_alone_and_in_the_mix(gpu, method, cap):
     srv.close()
 
 
-def test_symbols_demapped_into_slots(gpu):
-    """Requests of received symbols at their own noise levels, demapped by a
-    RowsDemapper into the ring slot: the slot's LLRs equal compute_llr's and
-    the decoded request equals decode() of those LLRs."""
+@pytest.mark.parametrize("batched", [False, True], ids=["per-request", "per-turn"])
+def test_symbols_demapped_into_slots(gpu, batched):
+    """Requests of received symbols at their own noise levels, demapped into
+    the ring slot by a RowsDemapper call each or queued on a
+    SegmentsDemapper and launched once a turn: the slot's LLRs equal
+    compute_llr's and the decoded request equals decode() of those LLRs."""
     import torch
 
     from commkit_amd import Constellation, compute_llr
-    from commkit_amd.mapping.demap import RowsDemapper
+    from commkit_amd.mapping.demap import RowsDemapper, SegmentsDemapper
     from commkit_amd.operating import waterfall_db
     from commkit_amd.workloads import make_frames
 
@@ -92,7 +97,8 @@ def test_symbols_demapped_into_slots(gpu):
     orders = (16, 64)
     method = MinSum(max_iterations=30)
     srv = Server(codes, method, capacities=[64, 64], max_inflight=2, max_launch_frames=24)
-    demap = {o: RowsDemapper(Constellation.qam(o), "maxlog", gpu) for o in orders}
+    kind = SegmentsDemapper if batched else RowsDemapper
+    demap = {o: kind(Constellation.qam(o), "maxlog", gpu) for o in orders}
     rng = np.random.default_rng(3)
     reqs = []
     for tag in range(20):
@@ -105,7 +111,14 @@ def test_symbols_demapped_into_slots(gpu):
         reqs.append((tag, c, R, order, fr.rx.contiguous(), inv, float(fr.noise_var[0])))
 
     def fill(slot, req):
-        demap[req[3]](req[4], req[5], slot.llrs)
+        if batched:
+            demap[req[3]].add(req[4], slot.llrs, inv_s2=req[5])
+        else:
+            demap[req[3]](req[4], req[5], slot.llrs)
+
+    def flush():
+        for d in demap.values():
+            d.flush()
 
     def check(slot, req):
         tag, c, R, order, rx, _, nv = req
@@ -116,5 +129,6 @@ def test_symbols_demapped_into_slots(gpu):
         assert torch.equal(slot.app, app), f"request {tag}: APP"
         assert torch.equal(slot.iterations, iters), f"request {tag}: iterations"
 
-    _serve(srv, reqs, fill, check, per_turn=2)
+    _serve(srv, reqs, fill, check, per_turn=5 if batched else 2,
+           before_step=flush if batched else None)  # fmt: skip
     srv.close()
