@@ -179,11 +179,15 @@ __device__ __forceinline__ void run_class(BatchDesc* __restrict__ d, int ci, flo
 // ONE: whether some class runs one lane per check row (0: none; 1: all such
 // classes have rows of at most kShortRow edges; 2: longer rows).  A lane of
 // a one-lane row holds all its slots in registers, which costs the kernel
-// registers (spills) that two-lane classes (rows of up to kMaxDegree / 2
-// slots a lane) should not pay, so each case is its own kernel.  Everything
-// is inlined: calls would move the live registers to scratch memory around
-// every row and syndrome check.
-template <bool SP, int ONE>
+// registers (spills) that two-lane classes should not pay, so each case is
+// its own kernel.  TWO: the slots a lane of a two-lane row holds, kMaxDegree
+// / 2 or kTwoShortSlots (rows of at most 28 edges): the longer row
+// instantiations alone cost spills (the mixed benchmark's launch: 5.53 ms
+// with 16 slots, 5.07 ms with 14).  Everything is inlined: calls would move
+// the live registers to scratch memory around every row and syndrome check.
+constexpr int kTwoShortSlots = 14;
+
+template <bool SP, int ONE, int TWO = kMaxDegree / 2>
 __global__ __launch_bounds__(kBatchThreads) void ldpc_batch(BatchDesc* __restrict__ d) {
     constexpr int kOneLaneSlots = ONE == 1 ? kShortRow : kMaxDegree;
     extern __shared__ float lds[];
@@ -195,9 +199,9 @@ __global__ __launch_bounds__(kBatchThreads) void ldpc_batch(BatchDesc* __restric
     for (int ci = 0; ci < nc; ++ci) {
         if (d->cls[ci].P == 2) {
             if (!SP && d->cls[ci].state_lds)
-                run_class<SP, 2, true, kMaxDegree / 2>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+                run_class<SP, 2, true, TWO>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
             else
-                run_class<SP, 2, false, kMaxDegree / 2>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
+                run_class<SP, 2, false, TWO>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
         } else if constexpr (ONE != 0) {
             if (!SP && d->cls[ci].state_lds)
                 run_class<SP, 1, true, kOneLaneSlots>(d, ci, lds, s_tab, s_need, s_frame, s_flag);
@@ -214,7 +218,8 @@ struct Layout {
     int F[kMaxClasses];
     int P[kMaxClasses];
     int S[kMaxClasses];  // check states in LDS
-    int one_lane;        // kernel variant: one-lane rows (0 none, 1 short, 2 long)
+    int variant;         // kernel: one-lane rows (0 none, 1 short, 2 long); 3: none,
+                         // and two-lane rows of at most 2 kTwoShortSlots edges
 };
 
 int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_method* m,
@@ -277,10 +282,15 @@ int32_t make_layout(const ck_batch_class* classes, int32_t num_classes, const ck
         L.slot_bytes = std::max(L.slot_bytes, static_cast<size_t>(F) * per_slot);
     }
     L.slot_bytes = (L.slot_bytes + 255) / 256 * 256;
-    L.one_lane = 0;
-    for (int i = 0; i < num_classes; ++i)
+    L.variant = 0;
+    bool two_short = true;
+    for (int i = 0; i < num_classes; ++i) {
         if (L.P[i] == 1)
-            L.one_lane = std::max(L.one_lane, classes[i].code->max_row_degree <= kShortRow ? 1 : 2);
+            L.variant = std::max(L.variant, classes[i].code->max_row_degree <= kShortRow ? 1 : 2);
+        else
+            two_short = two_short && classes[i].code->max_row_degree <= 2 * kTwoShortSlots;
+    }
+    if (L.variant == 0 && two_short) L.variant = 3;
     // One workgroup of twelve waves per CU: its registers (up to 168 VGPRs
     // a lane) leave no room for a second.
     L.grid = info.num_cus;
@@ -306,7 +316,7 @@ int32_t shape(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
         out->P[i] = L.P[i];
         out->S[i] = L.S[i];
     }
-    out->one_lane = L.one_lane;
+    out->variant = L.variant;
     return CK_OK;
 }
 
@@ -338,12 +348,15 @@ void describe(const ck_batch_class* classes, int32_t num_classes, const ck_ldpc_
 
 void allow_lds(size_t lds_bytes) {
     if (lds_bytes > 65536) {
-        const void* kernels[] = {reinterpret_cast<const void*>(ldpc_batch<false, 0>),
-                                 reinterpret_cast<const void*>(ldpc_batch<false, 1>),
-                                 reinterpret_cast<const void*>(ldpc_batch<false, 2>),
-                                 reinterpret_cast<const void*>(ldpc_batch<true, 0>),
-                                 reinterpret_cast<const void*>(ldpc_batch<true, 1>),
-                                 reinterpret_cast<const void*>(ldpc_batch<true, 2>)};
+        const void* kernels[] = {
+            reinterpret_cast<const void*>(ldpc_batch<false, 0>),
+            reinterpret_cast<const void*>(ldpc_batch<false, 1>),
+            reinterpret_cast<const void*>(ldpc_batch<false, 2>),
+            reinterpret_cast<const void*>(ldpc_batch<false, 0, kTwoShortSlots>),
+            reinterpret_cast<const void*>(ldpc_batch<true, 0>),
+            reinterpret_cast<const void*>(ldpc_batch<true, 1>),
+            reinterpret_cast<const void*>(ldpc_batch<true, 2>),
+            reinterpret_cast<const void*>(ldpc_batch<true, 0, kTwoShortSlots>)};
         for (const void* k : kernels)
             (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(lds_bytes));
@@ -351,21 +364,23 @@ void allow_lds(size_t lds_bytes) {
     }
 }
 
-int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, int one_lane,
+int32_t launch(void* workspace, int grid, size_t lds_bytes, bool sum_product, int variant,
                hipStream_t s) {
-    CK_REQUIRE(one_lane >= 0 && one_lane <= 2, "kernel variant %d", one_lane);
+    CK_REQUIRE(variant >= 0 && variant <= 3, "kernel variant %d", variant);
     BatchDesc* d = static_cast<BatchDesc*>(workspace);
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, d);
     if (int32_t st = ck::launched("batch reset")) return st;
     const dim3 g(static_cast<unsigned>(grid)), b(kBatchThreads);
     if (sum_product) {
-        if (one_lane == 0) hipLaunchKernelGGL((ldpc_batch<true, 0>), g, b, lds_bytes, s, d);
-        else if (one_lane == 1) hipLaunchKernelGGL((ldpc_batch<true, 1>), g, b, lds_bytes, s, d);
-        else hipLaunchKernelGGL((ldpc_batch<true, 2>), g, b, lds_bytes, s, d);
+        if (variant == 0) hipLaunchKernelGGL((ldpc_batch<true, 0>), g, b, lds_bytes, s, d);
+        else if (variant == 1) hipLaunchKernelGGL((ldpc_batch<true, 1>), g, b, lds_bytes, s, d);
+        else if (variant == 2) hipLaunchKernelGGL((ldpc_batch<true, 2>), g, b, lds_bytes, s, d);
+        else hipLaunchKernelGGL((ldpc_batch<true, 0, kTwoShortSlots>), g, b, lds_bytes, s, d);
     } else {
-        if (one_lane == 0) hipLaunchKernelGGL((ldpc_batch<false, 0>), g, b, lds_bytes, s, d);
-        else if (one_lane == 1) hipLaunchKernelGGL((ldpc_batch<false, 1>), g, b, lds_bytes, s, d);
-        else hipLaunchKernelGGL((ldpc_batch<false, 2>), g, b, lds_bytes, s, d);
+        if (variant == 0) hipLaunchKernelGGL((ldpc_batch<false, 0>), g, b, lds_bytes, s, d);
+        else if (variant == 1) hipLaunchKernelGGL((ldpc_batch<false, 1>), g, b, lds_bytes, s, d);
+        else if (variant == 2) hipLaunchKernelGGL((ldpc_batch<false, 2>), g, b, lds_bytes, s, d);
+        else hipLaunchKernelGGL((ldpc_batch<false, 0, kTwoShortSlots>), g, b, lds_bytes, s, d);
     }
     return ck::launched("batch decode");
 }
@@ -412,7 +427,7 @@ extern "C" int32_t ck_batch_prepare(const ck_batch_class* classes, int32_t num_c
         plan->frames_per_group[i] = i < num_classes ? L.F[i] : 0;
         plan->lanes_per_row[i] = i < num_classes ? L.P[i] : 0;
     }
-    plan->kernel_variant = L.one_lane;
+    plan->kernel_variant = L.variant;
     return CK_OK;
 }
 

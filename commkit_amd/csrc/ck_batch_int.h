@@ -20,7 +20,8 @@ struct Shape {
     int F[kMaxClasses];    // frame slots per workgroup, per class
     int P[kMaxClasses];    // lanes per check row, per class
     int S[kMaxClasses];    // 1: the class's min-sum check states live in LDS
-    int one_lane;          // kernel variant: one-lane rows (0 none, 1 short, 2 long)
+    int variant;           // kernel: one-lane rows (0 none, 1 short, 2 long); 3 none, and
+                           // every two-lane class's rows of at most 28 edges
 };
 
 // Bytes of the descriptor at the start of a launch's workspace; the

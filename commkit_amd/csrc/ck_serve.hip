@@ -375,7 +375,7 @@ extern "C" int32_t ck_serve_step(ck_serve* s, int64_t* launched_frames) {
                               hipMemcpyHostToDevice, l.stream),
                "hipMemcpyAsync(descriptor)");
         if (int32_t st = ckbatch::launch(l.workspace, sh.grid, sh.lds_bytes, s->method.rule == 1,
-                                         sh.one_lane, l.stream))
+                                         sh.variant, l.stream))
             return st;
         CK_HIP(hipEventRecord(l.done, l.stream), "hipEventRecord");
     }
