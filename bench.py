@@ -63,6 +63,8 @@ LEGS = ("c1", "c3", "c4", "c5")
 PROFILED_TRAFFIC = {
     "c2": {"frames": FRAMES, "bytes": 2.270e9, "demap_bytes": 0.393e9,
            "profile": "profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md"},
+    "c2_fused": {"frames": FRAMES, "bytes": None, "demap_bytes": None,
+                 "profile": "profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md"},
     "c5": {"frames": 3106, "bytes": 8.414e9, "demap_bytes": 0.3366e9,
            "profile": "profiles/ldpc_sum_product_shaped256_3106cw.md"},
 }  # fmt: skip
@@ -119,6 +121,9 @@ def _parse() -> argparse.Namespace:
                     help="serving: the requests admitted in a loop turn demapped together "
                          "(SegmentsDemapper, up to 16 requests a launch) instead of one "
                          "demapper launch per request")
+    ap.add_argument("--tail", choices=("fused", "two"), default="two",
+                    help="headline workload: demapping fused into the decoder's frame load "
+                         "(one launch), or the demapping kernel then the decoder")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device twice (counter calibration) and exit")
     return ap.parse_args()
@@ -466,7 +471,7 @@ def run_single(args, dev) -> tuple[dict, bool]:
     import torch
 
     from commkit_amd import Constellation, MinSum, metrics
-    from commkit_amd.coding.ldpc import _decode
+    from commkit_amd.coding.ldpc import _decode, _decode_symbols
     from commkit_amd.mapping.demap import llr_any
     from commkit_amd.operating import waterfall_db
     from commkit_amd.workloads import make_frames, seeded_code
@@ -480,6 +485,8 @@ def run_single(args, dev) -> tuple[dict, bool]:
     rx = fr.rx.contiguous()
     method = MinSum(max_iterations=50)
     state = {}
+    fused = args.tail == "fused"
+    sym = rx.reshape(F, -1)
 
     def demap():
         state["llr"] = llr_any(c, rx, nv, "maxlog")
@@ -487,7 +494,16 @@ def run_single(args, dev) -> tuple[dict, bool]:
     def dec():
         state["out"] = _decode(code, state["llr"].reshape(F, code.n), method)
 
-    elapsed, (demap_ms, decode_ms), step_ms = timed_steps(args.steps, args.warmup, [demap, dec])
+    def tail():
+        state["out"] = _decode_symbols(code, sym, c, nv, method)
+
+    if fused:
+        elapsed, (decode_ms,), step_ms = timed_steps(args.steps, args.warmup, [tail])
+        demap_ms = 0.0
+        demap()  # the LLRs the fused load computed, for the parity check (clock stopped)
+    else:
+        elapsed, (demap_ms, decode_ms), step_ms = timed_steps(args.steps, args.warmup,
+                                                              [demap, dec])  # fmt: skip
     bits, app, valid, iters = state["out"]
     iters_h = iters.cpu().numpy()
     # Scored by the library's own kernels, after the clock stopped.
@@ -496,8 +512,10 @@ def run_single(args, dev) -> tuple[dict, bool]:
     value = args.steps * F * code.k / elapsed / 1e9
     dec_bytes = decoder_bytes(code, iters_h)
     demap_bytes = rx.numel() * (8 + 4 * c.bits_per_symbol)
-    prof = PROFILED_TRAFFIC["c2"]
+    prof = PROFILED_TRAFFIC["c2_fused" if fused else "c2"]
     traffic = prof["bytes"] if F == prof["frames"] and args.esn0 is None else None
+    if fused:  # symbols in instead of LLRs
+        dec_bytes += rx.numel() * 8 - F * code.n * 4
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -518,27 +536,38 @@ def run_single(args, dev) -> tuple[dict, bool]:
             "esn0_db": esn0,
             "code": CODE,
             "constellation": f"{ORDER}-QAM",
+            "tail": ("one launch: max-log demapping fused into the decoder's frame load"
+                     if fused else "two launches: demapping kernel, then the decoder"),
         },
         "frame_latency_ms": pct(step_ms),
         "post_fec_ber": ber,
         "fer": fer,
         "mean_iterations": float(iters_h.mean()),
-        "kernels_ms": {"demap": round(demap_ms, 4), "decode": round(decode_ms, 4)},
-        "roofline": roofline(dec_bytes, decode_ms, "ldpc_qc_layered (decode)", traffic=traffic,
+        "kernels_ms": ({"demap+decode (fused)": round(decode_ms, 4)} if fused else
+                       {"demap": round(demap_ms, 4), "decode": round(decode_ms, 4)}),
+        "roofline": roofline(dec_bytes, decode_ms,
+                             "ldpc_qc_layered (demap fused into the load + decode)" if fused
+                             else "ldpc_qc_layered (decode)", traffic=traffic,
                              source=prof["profile"], limiter=LIMITER["c2"]),  # fmt: skip
-        "demap_roofline": roofline(demap_bytes, demap_ms, "llr_separable_kernel (demap)",
-                                   traffic=prof["demap_bytes"] if traffic is not None else None,
-                                   source=prof["profile"], limiter=LIMITER["c2_demap"]),
+    }
+    if not fused:
+        line["demap_roofline"] = roofline(
+            demap_bytes, demap_ms, "llr_separable_kernel (demap)",
+            traffic=prof["demap_bytes"] if traffic is not None else None,
+            source=prof["profile"], limiter=LIMITER["c2_demap"])  # fmt: skip
+    line.update({
         "tail_roofline": {
-            "achieved": round((demap_bytes + dec_bytes) / ((demap_ms + decode_ms) * 1e-3) / 1e9, 1),
+            "achieved": round((dec_bytes + (0 if fused else demap_bytes))
+                              / ((demap_ms + decode_ms) * 1e-3) / 1e9, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
         },
         "hbm_copy_GBs": round(copy_bandwidth(dev), 1),
-    }
+    })
     progress(leg="c2", value=line["value"], decode_ms=round(decode_ms, 4))
-    # Parity: the oracle decodes every frame's GPU LLRs; the GPU LLRs of a
-    # sample of symbols against the FP64 definition.
+    # Parity: the oracle decodes every frame's GPU LLRs (for the fused tail,
+    # those of the demapping kernel, which the fused load reproduces bit for
+    # bit); the GPU LLRs of a sample of symbols against the FP64 definition.
     llr_h = state["llr"].reshape(F, code.n).cpu().numpy()
     t0 = time.perf_counter()
     want = oracle_decode(code, llr_h, method)

@@ -41,6 +41,7 @@ SYMBOLS: dict[str, tuple[Any, list[Any]]] = {
     "ck_ldpc_workspace_bytes": (_i64, [_p, _p, _i64]),
     "ck_ldpc_decode": (_i32, [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _p]),
     "ck_ldpc_decode_static": (_i32, [_p, _p, _p, _p, _p, _p, _p, _i64, _p]),
+    "ck_ldpc_decode_symbols": (_i32, [_p, _p, _p, _p, _p, _f64, _p, _p, _p, _p, _p, _i64, _p, _p]),
     "ck_batch_workspace_bytes": (_i64, [_p, _i32, _p]),
     "ck_batch_prepare": (_i32, [_p, _i32, _p, _p, _i64, _p]),
     "ck_batch_launch": (_i32, [_p, _p]),

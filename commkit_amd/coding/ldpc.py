@@ -763,6 +763,79 @@ def method_struct(code: LDPCCode, method: MinSum | SumProduct) -> CkLdpcMethod:
     return m
 
 
+def _decode_symbols(code: LDPCCode, x: Any, constellation: Any, noise_var: Any,
+                    method: MinSum | SumProduct, *, tiles: Any = None):
+    """Demap and decode in one launch: ``x`` ``(R, n / k)`` complex64 device
+    symbols of a uniform separable constellation (max-log closed form, k
+    bits a symbol), ``noise_var`` a float or ``(R,)`` per frame; the same
+    ``(bits, app, valid, iterations)`` as ``_decode`` of the demapper's
+    LLRs, bit for bit (``ck_ldpc_decode_symbols``: the decoder's frame load
+    demaps, and the LLRs never pass through HBM).  Quasi-cyclic codes on
+    the layered kernel."""
+    import torch
+
+    from ..mapping.demap import axis_tables, closed_form
+
+    c = constellation
+    if (c._axes is None or not closed_form(c._axes, "maxlog")
+            or c._axes[0].levels.size != c._axes[1].levels.size):
+        raise ValueError("_decode_symbols: a uniform separable constellation with axes "
+                         "alike (closed-form max-log) is required")  # fmt: skip
+    k = c.bits_per_symbol
+    if code.n % k:
+        raise ValueError(f"_decode_symbols: n = {code.n} is not a whole number of {k}-bit symbols")
+    dev = x.device
+    R = int(x.shape[0])
+    S = code.n // k
+    if x.dtype != torch.complex64 or tuple(x.shape) != (R, S):
+        raise ValueError(f"_decode_symbols: x must be ({R}, {S}) complex64")
+    x = x.contiguous()
+    per_row = None
+    inv_s2 = 1.0
+    if np.ndim(noise_var) == 0:
+        inv_s2 = 1.0 / max(float(noise_var), 1e-20)
+    else:
+        nv = np.asarray(noise_var, dtype=np.float64).reshape(-1)
+        if nv.shape[0] != R or not np.all(np.isfinite(nv)) or np.any(nv < 0):
+            raise ValueError("_decode_symbols: one finite noise variance >= 0 per frame")
+        per_row = torch.from_numpy(1.0 / np.maximum(nv, 1e-20)).to(dev)
+    app = torch.empty((R, code.n), dtype=torch.float32, device=dev)
+    bits = torch.empty((R, code.k), dtype=torch.int8, device=dev)
+    iters = torch.empty(R, dtype=torch.int32, device=dev)
+    valid = torch.empty(R, dtype=torch.uint8, device=dev)
+    if R == 0:
+        return bits, app, valid.bool(), iters
+    lib = _native.lib()
+    with torch.cuda.device(dev):
+        dc = device_code(code, dev)
+        ms = method_struct(code, method)
+        if tiles is not None:
+            ms.kernel, ms.frames_per_group, ms.lanes_per_row, ms.state_in_lds = tiles
+        re_ax, im_ax, _keep = axis_tables(c, dev)
+        per = lib.ck_ldpc_workspace_bytes(ctypes.byref(dc.struct), ctypes.byref(ms), 1)
+        if per < 0 and tiles is None and (ms.kernel, ms.frames_per_group, ms.lanes_per_row,
+                                          ms.state_in_lds) != _DEFAULT_TILES:
+            ms.kernel, ms.frames_per_group, ms.lanes_per_row, ms.state_in_lds = _DEFAULT_TILES
+            per = lib.ck_ldpc_workspace_bytes(ctypes.byref(dc.struct), ctypes.byref(ms), 1)
+        if per < 0:
+            raise RuntimeError("ck_ldpc_workspace_bytes failed")
+        chunk = R if per == 0 else max(1, min(R, _GPU_WORKSPACE // max(per, 1)))
+        ws_bytes = lib.ck_ldpc_workspace_bytes(ctypes.byref(dc.struct), ctypes.byref(ms), chunk)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        stream = _native.stream_of(dev)
+        for r0 in range(0, R, chunk):
+            rc = min(chunk, R - r0)
+            _native.check(lib.ck_ldpc_decode_symbols(
+                ctypes.byref(dc.struct), ctypes.byref(ms), x.data_ptr() + 8 * r0 * S,
+                ctypes.byref(re_ax), ctypes.byref(im_ax), ctypes.c_double(inv_s2),
+                per_row.data_ptr() + 8 * r0 if per_row is not None else None,
+                app.data_ptr() + 4 * r0 * code.n, bits.data_ptr() + r0 * code.k,
+                iters.data_ptr() + 4 * r0, valid.data_ptr() + r0, rc,
+                ws.data_ptr() if ws_bytes else None, stream,
+            ))  # fmt: skip
+    return bits, app, valid.bool(), iters
+
+
 def _decode(code: LDPCCode, llrs: Any, method: MinSum | SumProduct, *, tiles: Any = None):
     """Decode ``(R, n)`` float32 device rows: ``(bits, app, valid, iterations)``."""
     import torch

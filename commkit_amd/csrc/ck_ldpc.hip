@@ -70,17 +70,25 @@ __global__ __launch_bounds__(kMaxQcThreads)
 __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void ldpc_qc_layered(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
-    unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm) {
+    unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm,
+    SymbolSource sym) {
     extern __shared__ float lds[];
     __shared__ int s_flag[kMaxFrames];
     __shared__ int2 s_tab[kTabEntries];
+    __shared__ DemapTab s_demap[2];
     const int N = c.num_variables;
     const int Z = c.lifting;
     const int mb = c.base_rows;
     const QcLane l = qc_lane<P>(Z, F, R);
     float* a = lds + static_cast<size_t>(l.f < F ? l.f : 0) * N;
     stage_slot_table(c, s_tab);
-    load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
+    if (sym.x != nullptr) {  // demapping fused into the load
+        stage_demap(s_demap, sym);
+        __syncthreads();
+        load_frames_demap(lds, sym, s_demap, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
+    } else {
+        load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
+    }
     __syncthreads();
 
     const QcLds lay = qc_lds(F, N, Z, c.num_checks, STATE_LDS);
@@ -770,12 +778,12 @@ void allow_lds(K kernel, size_t bytes) {
 template <bool SP, int P, bool SL, int SHORT = 0>
 void launch_qc_layered(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
                        signed char* bits, int* iters, unsigned char* valid, void* ws, long long R,
-                       const Params& prm, hipStream_t s) {
+                       const Params& prm, hipStream_t s, const SymbolSource& sym) {
     auto kernel = ldpc_qc_layered<SP, P, SL, SHORT>;
     allow_lds(kernel, p.lds_bytes);
     const long long groups = (R + p.F - 1) / p.F;
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(groups)), dim3(p.threads), p.lds_bytes,
-                       s, c, llr, app, bits, iters, valid, ws, R, p.F, prm);
+                       s, c, llr, app, bits, iters, valid, ws, R, p.F, prm, sym);
 }
 
 template <bool SP, int P>
@@ -824,16 +832,20 @@ extern "C" int64_t ck_ldpc_workspace_bytes(const ck_ldpc_code* code, const ck_ld
     return static_cast<int64_t>(round_up(p.ws_per_frame * static_cast<size_t>(R), 256));
 }
 
-extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method* method,
-                                  const float* llrs, float* app, int8_t* bits,
-                                  int32_t* iterations, uint8_t* valid, int64_t R, void* workspace,
-                                  void* stream) {
+// The decode of ck_ldpc_decode; with sym.x set (ck_ldpc_decode_symbols) the
+// frames come as symbols, demapped in the quasi-cyclic layered kernel's load.
+static int32_t decode_impl(const ck_ldpc_code* code, const ck_ldpc_method* method,
+                           const float* llrs, const SymbolSource& sym, float* app, int8_t* bits,
+                           int32_t* iterations, uint8_t* valid, int64_t R, void* workspace,
+                           void* stream) {
     Plan p{};
     if (int32_t st = make_plan(code, method, R, &p)) return st;
     CK_REQUIRE(R >= 0, "R = %lld < 0", static_cast<long long>(R));
     if (R == 0) return CK_OK;
     CK_REQUIRE(R <= 0x7fffffffLL, "R = %lld exceeds one launch", static_cast<long long>(R));
-    CK_REQUIRE(llrs && app && iterations && valid, "NULL input or output");
+    CK_REQUIRE((llrs || sym.x) && app && iterations && valid, "NULL input or output");
+    CK_REQUIRE(sym.x == nullptr || (p.kind == 2 && !p.flood),
+               "demapping fused into the decode needs the quasi-cyclic layered kernel");
     CK_REQUIRE(p.ws_per_frame == 0 || workspace != nullptr, "NULL workspace (%zu bytes per frame)",
                p.ws_per_frame);
     CK_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0, "workspace not 256-byte aligned");
@@ -854,23 +866,23 @@ extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method
             else if (p.P == 2) launch_qc_flooding<false, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
             else launch_qc_flooding<false, 4>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
         } else if (p.sp) {
-            if (p.P == 1) launch_qc_layered<true, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 2) launch_qc_layered<true, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else launch_qc_layered<true, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.P == 1) launch_qc_layered<true, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 2) launch_qc_layered<true, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else launch_qc_layered<true, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
         } else if (p.state_lds) {
-            if (p.short_rows == 2 && p.P == 1) launch_qc_layered<false, 1, true, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.short_rows == 1 && p.P == 1) launch_qc_layered<false, 1, true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 1) launch_qc_layered<false, 1, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 2) launch_qc_layered<false, 2, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else launch_qc_layered<false, 4, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows == 2 && p.P == 1) launch_qc_layered<false, 1, true, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.short_rows == 1 && p.P == 1) launch_qc_layered<false, 1, true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 1) launch_qc_layered<false, 1, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 2) launch_qc_layered<false, 2, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else launch_qc_layered<false, 4, true>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
         } else {
-            if (p.short_rows == 2 && p.P == 1) launch_qc_layered<false, 1, false, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.short_rows == 1 && p.P == 1) launch_qc_layered<false, 1, false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 1) launch_qc_layered<false, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 2) launch_qc_layered<false, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else launch_qc_layered<false, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.short_rows == 2 && p.P == 1) launch_qc_layered<false, 1, false, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.short_rows == 1 && p.P == 1) launch_qc_layered<false, 1, false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 1) launch_qc_layered<false, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 2 && p.short_rows) launch_qc_layered<false, 2, false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 2) launch_qc_layered<false, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else launch_qc_layered<false, 4, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
         }
         return ck::launched("ldpc quasi-cyclic decode");
     }
@@ -929,6 +941,49 @@ int32_t scratch_address(void** out) {
 }
 
 }  // namespace
+
+
+extern "C" int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method* method,
+                                  const float* llrs, float* app, int8_t* bits,
+                                  int32_t* iterations, uint8_t* valid, int64_t R, void* workspace,
+                                  void* stream) {
+    const SymbolSource none{};
+    return decode_impl(code, method, llrs, none, app, bits, iterations, valid, R, workspace,
+                       stream);
+}
+
+extern "C" int32_t ck_ldpc_decode_symbols(const ck_ldpc_code* code, const ck_ldpc_method* method,
+                                          const void* symbols, const ck_axis* re,
+                                          const ck_axis* im, double inv_s2,
+                                          const double* inv_rows, float* app, int8_t* bits,
+                                          int32_t* iterations, uint8_t* valid, int64_t R,
+                                          void* workspace, void* stream) {
+    CK_REQUIRE(code != nullptr && re != nullptr && im != nullptr, "NULL code or axis");
+    CK_REQUIRE(re->num_bits == im->num_bits && re->num_bits >= 1 &&
+                   re->num_bits <= kDemapMaxBits,
+               "fused demapping: axes of 1..%d bits each, alike (%d, %d)", kDemapMaxBits,
+               re->num_bits, im->num_bits);
+    for (const ck_axis* a : {re, im})
+        CK_REQUIRE(a->num_levels == (1 << a->num_bits) && a->levels && a->labels && a->lo &&
+                       a->hi && a->pos,
+                   "fused demapping: closed-form axis tables (levels, labels, opposite levels, "
+                   "positions) required");
+    const int k = 2 * re->num_bits;
+    CK_REQUIRE(code->n % k == 0, "fused demapping: n = %d is not a whole number of %d-bit symbols",
+               code->n, k);
+    CK_REQUIRE(inv_rows != nullptr || (inv_s2 > 0.0 && inv_s2 < 1e300),
+               "inv_s2 must be positive and finite");
+    CK_REQUIRE(R == 0 || symbols != nullptr, "NULL symbols");
+    SymbolSource sym{};
+    sym.x = static_cast<const float2*>(symbols);
+    sym.re = *re;
+    sym.im = *im;
+    sym.inv_s2 = inv_s2;
+    sym.inv_rows = inv_rows;
+    if (R == 0) sym.x = nullptr;
+    return decode_impl(code, method, nullptr, sym, app, bits, iterations, valid, R, workspace,
+                       stream);
+}
 
 extern "C" int32_t ck_ldpc_decode_static(const ck_ldpc_code* code, const ck_ldpc_method* method,
                                          const float* llrs, float* app, int8_t* bits,

@@ -235,6 +235,25 @@ int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method* method,
                        int32_t* iterations, uint8_t* valid, int64_t R,
                        void* workspace, void* stream);
 
+/* ck_ldpc_decode of frames given as channel symbols: symbols (R, n / k)
+ * interleaved complex float32 (k = re->num_bits + im->num_bits bits a
+ * symbol, both axes of 1..4 bits and alike, uniform levels with their
+ * closed-form tables lo/hi), demapped by max-log inside the decoder's frame
+ * load, so the LLRs never pass through device memory.  1/sigma^2 is inv_s2
+ * for every frame, or inv_rows[r] (R,) float64 device for frame r when
+ * inv_rows is not NULL.  Outputs, workspace and results are those of
+ * ck_ldpc_decode of the LLRs ck_llr_separable_maxlog (or
+ * ck_llr_separable_rows with closed = 1) writes for the same symbols, bit
+ * for bit.  Quasi-cyclic codes with the layered schedule only (CK_EINVAL
+ * otherwise).  No counterpart in the reference, whose receive tail runs
+ * compute_llr (commkit/mapping/llr.py:24-95) and then the decoder
+ * (commkit/coding/ldpc.py:566-587) as two launches. */
+int32_t ck_ldpc_decode_symbols(const ck_ldpc_code* code, const ck_ldpc_method* method,
+                               const void* symbols, const ck_axis* re, const ck_axis* im,
+                               double inv_s2, const double* inv_rows, float* app,
+                               int8_t* bits, int32_t* iterations, uint8_t* valid, int64_t R,
+                               void* workspace, void* stream);
+
 /* ck_ldpc_decode without a workspace argument and without allocating: the
  * workspace is a static device array of 64 MiB (each GPU the library runs
  * on holds one until the process ends), and frames are decoded in chunks
