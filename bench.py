@@ -61,21 +61,21 @@ LEGS = ("c1", "c3", "c4", "c5")
 # Counters cannot be read inside this process; --traffic reads them live in
 # child processes under rocprofv3 instead.
 PROFILED_TRAFFIC = {
-    "c2": {"frames": FRAMES, "bytes": 2.270e9, "demap_bytes": 0.393e9,
+    "c2": {"frames": FRAMES, "bytes": 2.120e9, "demap_bytes": 0.393e9,
            "profile": "profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md"},
     "c2_fused": {"frames": FRAMES, "bytes": None, "demap_bytes": None,
                  "profile": "profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md"},
-    "c5": {"frames": 3106, "bytes": 8.414e9, "demap_bytes": 0.3366e9,
+    "c5": {"frames": 3106, "bytes": 8.413e9, "demap_bytes": 0.3366e9,
            "profile": "profiles/ldpc_sum_product_shaped256_3106cw.md"},
 }  # fmt: skip
 
 # What the counters of the decoder kernels show limits them (the profiles
 # above): VALU issue and waits, not HBM bandwidth.
 LIMITER = {
-    "c2": "VALU issue and barrier/LDS waits (SQ_INSTS_VALU fills ~54% of SIMD cycles, "
-          "SQ_WAIT_ANY 47% of wave cycles); APP and check state live in LDS, so HBM "
+    "c2": "VALU issue and barrier/LDS waits (SQ_INSTS_VALU fills ~56% of SIMD cycles, "
+          "SQ_WAIT_ANY 45% of wave cycles); APP and check state live in LDS, so HBM "
           "traffic is a fifth of the algorithmic bytes",
-    "c5": "VALU issue, the phi evaluations first (SQ_INSTS_VALU fills ~66% of SIMD cycles, "
+    "c5": "VALU issue, the phi evaluations first (SQ_INSTS_VALU fills ~65% of SIMD cycles, "
           "73% by SQ_ACTIVE_INST_VALU; 6.5 transcendentals per edge and iteration)",
     "c2_demap": "VALU issue (~70% of SIMD cycles) and load latency (60% of wave cycles "
                 "waiting); a device copy of the same bytes runs at 5.3 TB/s",
