@@ -72,26 +72,25 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm,
     SymbolSource sym) {
-    extern __shared__ float lds[];
+    // 16-byte aligned: float4 frame loads, the fused demapping's tables
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ int s_flag[kMaxFrames];
     __shared__ int2 s_tab[kTabEntries];
-    __shared__ DemapTab s_demap[2];
     const int N = c.num_variables;
     const int Z = c.lifting;
     const int mb = c.base_rows;
     const QcLane l = qc_lane<P>(Z, F, R);
     float* a = lds + static_cast<size_t>(l.f < F ? l.f : 0) * N;
+    const QcLds lay = qc_lds(F, N, Z, c.num_checks, STATE_LDS);
     stage_slot_table(c, s_tab);
-    if (sym.x != nullptr) {  // demapping fused into the load
-        stage_demap(s_demap, sym);
-        __syncthreads();
-        load_frames_demap(lds, sym, s_demap, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
+    if (sym.x != nullptr) {  // demapping fused into the load; tables past the APPs
+        load_frames_demap(lds, sym, lds + lay.bits, static_cast<long long>(blockIdx.x) * F, F, R,
+                          c.n, N);
     } else {
         load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
     }
     __syncthreads();
 
-    const QcLds lay = qc_lds(F, N, Z, c.num_checks, STATE_LDS);
     unsigned* bits = reinterpret_cast<unsigned*>(lds + lay.bits);
     const bool lane_ok = l.f < F;
     CheckState* st_base = nullptr;
@@ -846,6 +845,21 @@ static int32_t decode_impl(const ck_ldpc_code* code, const ck_ldpc_method* metho
     CK_REQUIRE((llrs || sym.x) && app && iterations && valid, "NULL input or output");
     CK_REQUIRE(sym.x == nullptr || (p.kind == 2 && !p.flood),
                "demapping fused into the decode needs the quasi-cyclic layered kernel");
+    if (sym.x != nullptr) {
+        // The demapping tables go past the frames' APPs, over the sign
+        // bitmaps and check states (unused until the load ends).
+        ck::DeviceInfo info{};
+        if (int32_t st = ck::device_info(&info)) return st;
+        const size_t need = qc_lds(p.F, code->num_variables, code->lifting, code->num_checks,
+                                   p.state_lds).bits * 4 +
+                            demap_table_bytes(sym.re.num_bits);
+        CK_REQUIRE(need + kStaticLds + 1024 <= info.lds_per_block,
+                   "fused demapping: %zu bytes of LDS per workgroup exceed the device's %zu", need,
+                   info.lds_per_block);
+        p.lds_bytes = std::max(p.lds_bytes, need);
+        if (p.short_rows == 2 && 2 * (p.lds_bytes + kStaticLds) > info.lds_per_block)
+            p.short_rows = 1;  // one workgroup per CU
+    }
     CK_REQUIRE(p.ws_per_frame == 0 || workspace != nullptr, "NULL workspace (%zu bytes per frame)",
                p.ws_per_frame);
     CK_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0, "workspace not 256-byte aligned");

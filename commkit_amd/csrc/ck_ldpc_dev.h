@@ -5,6 +5,7 @@
 // Arithmetic contract: see ck_ldpc.hip.
 #pragma once
 
+#include "ck_closed.h"
 #include "ck_common.h"
 
 namespace ckdev {
@@ -15,7 +16,7 @@ constexpr int kMaxFrames = 64;   // frames per QC workgroup
 constexpr int kMaxDegree = 32;   // check degree of the compressed min-sum state
 constexpr int kMaxSlots = 480;   // nonzero circulants of a QC code the QC kernels take
 constexpr int kTabEntries = kMaxSlots + kMaxDegree;  // LDS slot table incl. padding
-constexpr size_t kStaticLds = 8 * kTabEntries + 4 * kMaxFrames + 2 * 568;  // static LDS of the QC kernels (slot table, flags, fused demapping tables)
+constexpr size_t kStaticLds = 8 * kTabEntries + 4 * kMaxFrames;  // static LDS of the QC kernels (slot table, flags)
 constexpr float kPhiMin = 1e-7f;
 constexpr float kPhiMax = 30.0f;
 
@@ -173,14 +174,14 @@ __device__ inline void load_frame(float* a, const float* __restrict__ src, int n
 }
 
 // Demapping fused into the frame load: a frame's channel LLRs computed from
-// its received symbols straight into LDS, by the closed-form max-log of a
-// uniform separable lattice (the kernel llr_separable_kernel<KB, true> of
-// the demapper, restated with the bit count at run time, operation for
-// operation, so the LLRs are bit for bit the demapper's).  One symbol holds
-// k = 2 kb consecutive code bits; with this source the decoder never reads
-// or writes LLRs in HBM.
+// its received symbols straight into LDS by the closed-form max-log of a
+// uniform separable lattice, with the demapper's cell tables and per-axis
+// function (ck_closed.h), so the LLRs are bit for bit the demapper's.  One
+// symbol holds k = 2 kb consecutive code bits; with this source the decoder
+// never reads or writes LLRs in HBM.  The tables are staged in the dynamic
+// LDS past the frames' APPs (the sign bitmaps and check states there are
+// not used before the load ends).
 constexpr int kDemapMaxBits = 4;  // per axis: up to 256-QAM
-constexpr int kDemapMaxLevels = 1 << kDemapMaxBits;
 
 struct SymbolSource {
     const float2* x;         // (R, n / k) complex64 symbols; NULL: the frames are LLRs
@@ -189,94 +190,28 @@ struct SymbolSource {
     const double* inv_rows;  // (R,) 1 / sigma^2 per frame, or NULL
 };
 
-struct DemapTab {
-    double lev[kDemapMaxLevels];
-    float levf[kDemapMaxLevels];
-    int lab[kDemapMaxLevels];
-    short lo[kDemapMaxBits][kDemapMaxLevels];
-    short hi[kDemapMaxBits][kDemapMaxLevels];
-    int pos[kDemapMaxBits];
-    int kb;
-    float inv_step;
-};
-static_assert(2 * sizeof(DemapTab) <= kStaticLds - 8 * kTabEntries - 4 * kMaxFrames,
-              "the QC kernels' static LDS budget covers the fused demapping tables");
-
-// Both axes' tables into LDS (the caller's barrier follows).
-__device__ inline void stage_demap(DemapTab* tab, const SymbolSource& src) {
-    for (int ax = 0; ax < 2; ++ax) {
-        const ck_axis& a = ax == 0 ? src.re : src.im;
-        DemapTab& t = tab[ax];
-        const int kb = a.num_bits;
-        const int L = 1 << kb;
-        for (int i = threadIdx.x; i < L; i += blockDim.x) {
-            t.lev[i] = a.levels[i];
-            t.levf[i] = static_cast<float>(a.levels[i]);
-            t.lab[i] = a.labels[i];
-        }
-        for (int i = threadIdx.x; i < kb * L; i += blockDim.x) {
-            t.lo[i / L][i % L] = static_cast<short>(a.lo[i]);
-            t.hi[i / L][i % L] = static_cast<short>(a.hi[i]);
-        }
-        if (threadIdx.x < static_cast<unsigned>(kb)) t.pos[threadIdx.x] = a.pos[threadIdx.x];
-        if (threadIdx.x == 0) {
-            t.kb = kb;
-            t.inv_step = 1.0f / (static_cast<float>(a.levels[1]) - static_cast<float>(a.levels[0]));
-        }
-    }
-}
-
-// One axis of symbol component x: its kb LLRs (+0.0f, as the frame load
-// takes LLRs) into row[pos[b]].  The nearest level against the nearer of
-// the two levels just outside its run of equal bits, in float64.
-template <int KB>
-__device__ __forceinline__ void demap_axis(const DemapTab& t, float x, double inv_s2, float* row) {
-    constexpr int kb = KB;
-    constexpr int L = 1 << KB;
-    int s = __float2int_rn((x - t.levf[0]) * t.inv_step);
-    s = min(max(s, 0), L - 1);
-    const double xd = x;
-    double d = xd - t.lev[s];
-    const int s2 = d > 0.0 ? s + 1 : s - 1;
-    if (s2 >= 0 && s2 < L) {
-        const double d2 = xd - t.lev[s2];
-        if (fabs(d2) < fabs(d)) {
-            s = s2;
-            d = d2;
-        }
-    }
-    const double ls = t.lev[s];
-    const int lab = t.lab[s];
-#pragma unroll
-    for (int b = 0; b < kb; ++b) {
-        const int jl = t.lo[b][s];
-        const int jh = t.hi[b][s];
-        int j;
-        if (jl < 0) {
-            j = jh;
-        } else if (jh < 0) {
-            j = jl;
-        } else {
-            j = (fabs(xd - t.lev[jh]) < fabs(xd - t.lev[jl])) ? jh : jl;
-        }
-        j = j < 0 ? s : j;
-        const double lo = t.lev[j];
-        const double diff = (ls - lo) * (d + (xd - lo)) * inv_s2;
-        const float v = static_cast<float>(diff);
-        row[t.pos[b]] = (((lab >> (kb - 1 - b)) & 1) ? -v : v) + 0.0f;
-    }
+// LDS bytes of both axes' tables for kb bits per axis.
+__host__ __device__ constexpr size_t demap_table_bytes(int kb) {
+    return kb == 1   ? 2 * sizeof(ck::ClosedAxis<1>)
+           : kb == 2 ? 2 * sizeof(ck::ClosedAxis<2>)
+           : kb == 3 ? 2 * sizeof(ck::ClosedAxis<3>)
+                     : 2 * sizeof(ck::ClosedAxis<4>);
 }
 
 // Frames [frame0, frame0 + F) demapped into LDS (zeros for the punctured
 // tail and for frames past R); a lane's next kDemapBatch symbols are loaded
-// before the first is demapped.  KB bits per axis.
+// before the first is demapped.  KB bits per axis; `tables` (16-byte
+// aligned LDS) takes both axes' tables.
 constexpr int kDemapBatch = 4;
 
 template <int KB>
-__device__ inline void load_frames_demap_kb(float* lds, const SymbolSource& src,
-                                            const DemapTab* tab, long long frame0, int F,
-                                            long long R, int n, int N) {
+__device__ inline void load_frames_demap_kb(float* lds, const SymbolSource& src, void* tables,
+                                            long long frame0, int F, long long R, int n, int N) {
     constexpr int k = 2 * KB;
+    ck::ClosedAxis<KB>* tab = static_cast<ck::ClosedAxis<KB>*>(tables);
+    ck::stage_closed<KB>(tab[0], src.re);
+    ck::stage_closed<KB>(tab[1], src.im);
+    __syncthreads();
     const int S = n / k;  // symbols per frame
     const int total = F * S;
     const int step = static_cast<int>(blockDim.x);
@@ -299,8 +234,8 @@ __device__ inline void load_frames_demap_kb(float* lds, const SymbolSource& src,
             const int f = q / S;
             float* row = lds + static_cast<size_t>(f) * N + static_cast<size_t>(q - f * S) * k;
             if (frame0 + f < R) {
-                demap_axis<KB>(tab[0], v[u].x, inv[u], row);
-                demap_axis<KB>(tab[1], v[u].y, inv[u], row);
+                ck::closed_axis_llrs<KB, true>(tab[0], v[u].x, inv[u], row);
+                ck::closed_axis_llrs<KB, true>(tab[1], v[u].y, inv[u], row);
             } else {
 #pragma unroll
                 for (int b = 0; b < k; ++b) row[b] = 0.0f;
@@ -314,13 +249,13 @@ __device__ inline void load_frames_demap_kb(float* lds, const SymbolSource& src,
     }
 }
 
-__device__ inline void load_frames_demap(float* lds, const SymbolSource& src, const DemapTab* tab,
+__device__ inline void load_frames_demap(float* lds, const SymbolSource& src, void* tables,
                                          long long frame0, int F, long long R, int n, int N) {
     switch (src.re.num_bits) {  // a kernel argument: a scalar branch
-        case 1: load_frames_demap_kb<1>(lds, src, tab, frame0, F, R, n, N); break;
-        case 2: load_frames_demap_kb<2>(lds, src, tab, frame0, F, R, n, N); break;
-        case 3: load_frames_demap_kb<3>(lds, src, tab, frame0, F, R, n, N); break;
-        default: load_frames_demap_kb<4>(lds, src, tab, frame0, F, R, n, N); break;
+        case 1: load_frames_demap_kb<1>(lds, src, tables, frame0, F, R, n, N); break;
+        case 2: load_frames_demap_kb<2>(lds, src, tables, frame0, F, R, n, N); break;
+        case 3: load_frames_demap_kb<3>(lds, src, tables, frame0, F, R, n, N); break;
+        default: load_frames_demap_kb<4>(lds, src, tables, frame0, F, R, n, N); break;
     }
 }
 

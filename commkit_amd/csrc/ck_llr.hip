@@ -35,6 +35,7 @@
 // k LLRs into an LDS tile of kThreads rows of stride (k | 1) floats (odd, so
 // lane t's row starts in its own bank) and the workgroup stores the tile to
 // HBM contiguously, as 16-byte vectors where the tile is whole.
+#include "ck_closed.h"
 #include "ck_common.h"
 
 namespace {
@@ -596,14 +597,59 @@ __device__ __forceinline__ void axis_levels(const AxisTab<KB>& t, float x, float
 #define CK_SEPARABLE_ARGS \
     x, n, re, im, inv_s2f, inv_s2, exact, out, inv_rows, row_len, perm, period, limit
 
+// The LDS tables of a separable kernel: the cell tables of the closed form
+// (ck_closed.h) for axes of up to kClosedTableBits bits, else the per-level
+// tables (level metrics, and the closed form of wider axes).
+template <int KB, bool CLOSED>
+constexpr bool kCellTables = CLOSED && KB <= ck::kClosedTableBits;
+
+template <int KB, bool CLOSED, bool CELLS = kCellTables<KB, CLOSED>>
+struct SeparableLds {
+    AxisTab<KB> tab[2];
+};
+
+template <int KB, bool CLOSED>
+struct SeparableLds<KB, CLOSED, true> {
+    ck::ClosedAxis<KB> tab[2];
+};
+
+// Both axes' tables into LDS, then a barrier.
+template <int KB, bool CLOSED>
+__device__ __forceinline__ void stage_separable(SeparableLds<KB, CLOSED>& t, const ck_axis& re,
+                                                const ck_axis& im) {
+    if constexpr (kCellTables<KB, CLOSED>) {
+        ck::stage_closed<KB>(t.tab[0], re);
+        ck::stage_closed<KB>(t.tab[1], im);
+    } else {
+        stage_axis<KB>(t.tab[0], re, CLOSED);
+        stage_axis<KB>(t.tab[1], im, CLOSED);
+    }
+    __syncthreads();
+}
+
+// 1 / level step of both axes (per-level tables only: the cell tables carry
+// their own coordinate map).
+template <int KB, bool CLOSED>
+__device__ __forceinline__ float2 level_steps(const SeparableLds<KB, CLOSED>& t) {
+    if constexpr (kCellTables<KB, CLOSED>) {
+        return make_float2(0.0f, 0.0f);
+    } else {
+        return make_float2(1.0f / (t.tab[0].levf[1] - t.tab[0].levf[0]),
+                           1.0f / (t.tab[1].levf[1] - t.tab[1].levf[0]));
+    }
+}
+
 // The k LLRs of one symbol v into row: both axes of the staged tables.
 template <int KB, bool CLOSED>
-__device__ __forceinline__ void separable_symbol(AxisTab<KB>* tab, float2 v, float inv_s2f,
-                                                 double inv_s2, float inv_step0, float inv_step1,
+__device__ __forceinline__ void separable_symbol(SeparableLds<KB, CLOSED>& t, float2 v,
+                                                 float inv_s2f, double inv_s2, float2 inv_step,
                                                  int exact, float* row) {
-    if (CLOSED) {
-        axis_closed<KB>(tab[0], v.x, inv_s2, inv_step0, row);
-        axis_closed<KB>(tab[1], v.y, inv_s2, inv_step1, row);
+    if constexpr (kCellTables<KB, CLOSED>) {
+        ck::closed_axis_llrs<KB>(t.tab[0], v.x, inv_s2, row);
+        ck::closed_axis_llrs<KB>(t.tab[1], v.y, inv_s2, row);
+    } else if constexpr (CLOSED) {
+        axis_closed<KB>(t.tab[0], v.x, inv_s2, inv_step.x, row);
+        axis_closed<KB>(t.tab[1], v.y, inv_s2, inv_step.y, row);
     } else {
         // The table index is opaque to the compiler, so the level tables are
         // read from LDS per symbol (broadcast reads) instead of held in
@@ -611,21 +657,18 @@ __device__ __forceinline__ void separable_symbol(AxisTab<KB>* tab, float2 v, flo
         // axis would spill.
         int z = 0;
         asm volatile("" : "+v"(z));
-        axis_levels<KB>(tab[z], v.x, inv_s2f, inv_s2, inv_step0, exact != 0, row);
-        axis_levels<KB>(tab[z + 1], v.y, inv_s2f, inv_s2, inv_step1, exact != 0, row);
+        axis_levels<KB>(t.tab[z], v.x, inv_s2f, inv_s2, inv_step.x, exact != 0, row);
+        axis_levels<KB>(t.tab[z + 1], v.y, inv_s2f, inv_s2, inv_step.y, exact != 0, row);
     }
 }
 
 template <int KB, bool CLOSED>
 __device__ __forceinline__ void separable_body(CK_SEPARABLE_PARAMS) {
     constexpr int K = 2 * KB;
-    __shared__ AxisTab<KB> tab[2];
+    __shared__ SeparableLds<KB, CLOSED> lt;
     __shared__ float tile[kThreads * (K | 1)];
-    stage_axis<KB>(tab[0], re, CLOSED);
-    stage_axis<KB>(tab[1], im, CLOSED);
-    __syncthreads();
-    const float inv_step0 = 1.0f / (tab[0].levf[1] - tab[0].levf[0]);
-    const float inv_step1 = 1.0f / (tab[1].levf[1] - tab[1].levf[0]);
+    stage_separable<KB, CLOSED>(lt, re, im);
+    const float2 inv_step = level_steps<KB, CLOSED>(lt);
     float* row = tile + threadIdx.x * tile_stride(K);
     for (long long base = static_cast<long long>(blockIdx.x) * kThreads; base < n;
          base += static_cast<long long>(gridDim.x) * kThreads) {
@@ -640,8 +683,7 @@ __device__ __forceinline__ void separable_body(CK_SEPARABLE_PARAMS) {
                 inv_s2 = inv_rows[r];
                 inv_s2f = static_cast<float>(inv_s2);
             }
-            separable_symbol<KB, CLOSED>(tab, v, inv_s2f, inv_s2, inv_step0, inv_step1, exact,
-                                         row);
+            separable_symbol<KB, CLOSED>(lt, v, inv_s2f, inv_s2, inv_step, exact, row);
         }
         __syncthreads();
         const long long left = n - base;
@@ -696,13 +738,10 @@ template <int KB, bool CLOSED>
 __device__ __forceinline__ void segments_body(const Segments& sg, const ck_axis& re,
                                               const ck_axis& im, int exact) {
     constexpr int K = 2 * KB;
-    __shared__ AxisTab<KB> tab[2];
+    __shared__ SeparableLds<KB, CLOSED> lt;
     __shared__ float tile[kThreads * (K | 1)];
-    stage_axis<KB>(tab[0], re, CLOSED);
-    stage_axis<KB>(tab[1], im, CLOSED);
-    __syncthreads();
-    const float inv_step0 = 1.0f / (tab[0].levf[1] - tab[0].levf[0]);
-    const float inv_step1 = 1.0f / (tab[1].levf[1] - tab[1].levf[0]);
+    stage_separable<KB, CLOSED>(lt, re, im);
+    const float2 inv_step = level_steps<KB, CLOSED>(lt);
     float* row = tile + threadIdx.x * tile_stride(K);
     const long long tiles = sg.tile_end[sg.count - 1];
     for (long long tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
@@ -722,8 +761,8 @@ __device__ __forceinline__ void segments_body(const Segments& sg, const ck_axis&
                                                            static_cast<unsigned>(len))
                                   : i / len];
             }
-            separable_symbol<KB, CLOSED>(tab, v, static_cast<float>(inv_s2), inv_s2, inv_step0,
-                                         inv_step1, exact, row);
+            separable_symbol<KB, CLOSED>(lt, v, static_cast<float>(inv_s2), inv_s2, inv_step,
+                                         exact, row);
         }
         __syncthreads();
         const int count = n - base < kThreads ? static_cast<int>(n - base) : kThreads;
