@@ -5,9 +5,14 @@ Headline workload (one GPU): the rate-4/5 QC-LDPC code drawn from seed 104
 n = 18000, k = 14400); 4096 codewords of uniform info bits, encoded, Gray
 64-QAM, complex64, AWGN at 0.3 dB above the code's measured waterfall
 (commkit_amd.operating).  A step is one pass of the hot path over the batch
-already resident in HBM: max-log float32 LLRs of the 12,288,000 symbols
-(``compute_llr``), then layered normalized min-sum (scale 0.875, at most 50
-iterations, syndrome early stop) of the 4096 frames (``decode``).
+already resident in HBM: max-log float32 LLRs of the 12,288,000 symbols,
+then layered normalized min-sum (scale 0.875, at most 50 iterations,
+syndrome early stop) of the 4096 frames.  The headline tail is one launch,
+the demapping done in the decoder's frame load (``ck_ldpc_decode_symbols``);
+the same symbols through two launches (``compute_llr``, then ``decode``)
+are timed too and reported under ``other_tail``, and both tails' outputs
+are checked equal.  Steps are queued back to back (no host synchronization
+between them); frame latency is a step's time on the GPU.
 
 The same JSON line carries, under ``configs``, the other workloads of the
 tail, each measured in this run with its own roofline, CPU baseline and
@@ -121,9 +126,12 @@ def _parse() -> argparse.Namespace:
                     help="serving: the requests admitted in a loop turn demapped together "
                          "(SegmentsDemapper, up to 16 requests a launch) instead of one "
                          "demapper launch per request")
-    ap.add_argument("--tail", choices=("fused", "two"), default="two",
-                    help="headline workload: demapping fused into the decoder's frame load "
-                         "(one launch), or the demapping kernel then the decoder")
+    ap.add_argument("--tail", choices=("fused", "two"), default="fused",
+                    help="headline tail: demapping fused into the decoder's frame load "
+                         "(one launch), or the demapping kernel then the decoder; the other "
+                         "is timed too and reported under other_tail")
+    ap.add_argument("--one-tail", action="store_true",
+                    help="time only the headline tail (profiling runs: one kernel mix)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device twice (counter calibration) and exit")
     return ap.parse_args()
@@ -322,8 +330,12 @@ def copy_calibration(dev) -> None:
 def timed_steps(steps: int, warmup: int, stages) -> tuple[float, list, list]:
     """Runs ``warmup`` untimed then ``steps`` timed passes of ``stages`` (a
     list of callables run in order on torch's current stream), with HIP
-    events around each stage.  Returns (wall seconds of the timed steps,
-    per-stage mean ms, per-step wall ms)."""
+    events around each stage.  The steps are queued back to back, as a
+    receiver streams batches (the host prepares step s + 1 while the GPU
+    runs step s; no host synchronization between steps), and the clock is
+    synchronized on both sides.  Returns (wall seconds of the timed steps,
+    per-stage mean ms, per-step ms on the GPU: from the step's first event
+    to its last, the time its frames spend being processed)."""
     import torch
 
     for _ in range(warmup):
@@ -333,21 +345,18 @@ def timed_steps(steps: int, warmup: int, stages) -> tuple[float, list, list]:
     stream = torch.cuda.current_stream()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
           for _ in range(steps)]  # fmt: skip
-    step_ms = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):
-        ts = time.perf_counter()
         ev[s][0].record(stream)
         for j, fn in enumerate(stages):
             fn()
             ev[s][j + 1].record(stream)
-        ev[s][-1].synchronize()
-        step_ms.append((time.perf_counter() - ts) * 1e3)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     stage_ms = [float(np.mean([e[j].elapsed_time(e[j + 1]) for e in ev]))
                 for j in range(len(stages))]  # fmt: skip
+    step_ms = [e[0].elapsed_time(e[-1]) for e in ev]
     return elapsed, stage_ms, step_ms
 
 
@@ -492,30 +501,56 @@ def run_single(args, dev) -> tuple[dict, bool]:
         state["llr"] = llr_any(c, rx, nv, "maxlog")
 
     def dec():
-        state["out"] = _decode(code, state["llr"].reshape(F, code.n), method)
+        state["two"] = _decode(code, state["llr"].reshape(F, code.n), method)
 
     def tail():
-        state["out"] = _decode_symbols(code, sym, c, nv, method)
+        state["fused"] = _decode_symbols(code, sym, c, nv, method)
 
-    if fused:
-        elapsed, (decode_ms,), step_ms = timed_steps(args.steps, args.warmup, [tail])
-        demap_ms = 0.0
+    # Both tails on the same symbols: one launch (the demapping in the
+    # decoder's frame load) and two (demapping kernel, then the decoder);
+    # the headline is args.tail's, the other is reported beside it.
+    runs = {}
+    modes = ("fused", "two") if fused else ("two", "fused")
+    for mode in modes[:1] if args.one_tail else modes:
+        if mode == "fused":
+            el, (dms,), sms = timed_steps(args.steps, args.warmup, [tail])
+            runs[mode] = (el, 0.0, dms, sms)
+        else:
+            el, (mms, dms), sms = timed_steps(args.steps, args.warmup, [demap, dec])
+            runs[mode] = (el, mms, dms, sms)
+    head = "fused" if fused else "two"
+    elapsed, demap_ms, decode_ms, step_ms = runs[head]
+    if args.one_tail and fused:
         demap()  # the LLRs the fused load computed, for the parity check (clock stopped)
-    else:
-        elapsed, (demap_ms, decode_ms), step_ms = timed_steps(args.steps, args.warmup,
-                                                              [demap, dec])  # fmt: skip
+        dec()
+    state["out"] = state[head]
+    tails_equal = all(torch.equal(x, y) for x, y in zip(state["fused"], state["two"])) \
+        if "fused" in state and "two" in state else True
     bits, app, valid, iters = state["out"]
     iters_h = iters.cpu().numpy()
     # Scored by the library's own kernels, after the clock stopped.
     ber = metrics.ber(bits.reshape(-1), fr.info.reshape(-1))
     fer = metrics.fer(bits.reshape(-1), fr.info.reshape(-1), info_bits=code.k)
     value = args.steps * F * code.k / elapsed / 1e9
-    dec_bytes = decoder_bytes(code, iters_h)
+    base_bytes = decoder_bytes(code, iters_h)
     demap_bytes = rx.numel() * (8 + 4 * c.bits_per_symbol)
-    prof = PROFILED_TRAFFIC["c2_fused" if fused else "c2"]
-    traffic = prof["bytes"] if F == prof["frames"] and args.esn0 is None else None
-    if fused:  # symbols in instead of LLRs
-        dec_bytes += rx.numel() * 8 - F * code.n * 4
+    fused_bytes = base_bytes + rx.numel() * 8 - F * code.n * 4  # symbols in instead of LLRs
+    measured = F == FRAMES and args.esn0 is None
+
+    def decoder_roofline(mode: str, ms: float) -> dict:
+        prof = PROFILED_TRAFFIC["c2_fused" if mode == "fused" else "c2"]
+        return roofline(fused_bytes if mode == "fused" else base_bytes, ms,
+                        "ldpc_qc_layered (demap fused into the load + decode)" if mode == "fused"
+                        else "ldpc_qc_layered (decode)",
+                        traffic=prof["bytes"] if measured else None, source=prof["profile"],
+                        limiter=LIMITER["c2"])  # fmt: skip
+
+    def demapper_roofline(ms: float) -> dict:
+        prof = PROFILED_TRAFFIC["c2"]
+        return roofline(demap_bytes, ms, "llr_separable_kernel (demap)",
+                        traffic=prof["demap_bytes"] if measured else None,
+                        source=prof["profile"], limiter=LIMITER["c2_demap"])  # fmt: skip
+
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -545,25 +580,34 @@ def run_single(args, dev) -> tuple[dict, bool]:
         "mean_iterations": float(iters_h.mean()),
         "kernels_ms": ({"demap+decode (fused)": round(decode_ms, 4)} if fused else
                        {"demap": round(demap_ms, 4), "decode": round(decode_ms, 4)}),
-        "roofline": roofline(dec_bytes, decode_ms,
-                             "ldpc_qc_layered (demap fused into the load + decode)" if fused
-                             else "ldpc_qc_layered (decode)", traffic=traffic,
-                             source=prof["profile"], limiter=LIMITER["c2"]),  # fmt: skip
+        "roofline": decoder_roofline(head, decode_ms),
     }
     if not fused:
-        line["demap_roofline"] = roofline(
-            demap_bytes, demap_ms, "llr_separable_kernel (demap)",
-            traffic=prof["demap_bytes"] if traffic is not None else None,
-            source=prof["profile"], limiter=LIMITER["c2_demap"])  # fmt: skip
-    line.update({
-        "tail_roofline": {
-            "achieved": round((dec_bytes + (0 if fused else demap_bytes))
-                              / ((demap_ms + decode_ms) * 1e-3) / 1e9, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-        },
-        "hbm_copy_GBs": round(copy_bandwidth(dev), 1),
-    })
+        line["demap_roofline"] = demapper_roofline(demap_ms)
+    line["tail_roofline"] = {
+        "achieved": round((fused_bytes if fused else base_bytes + demap_bytes)
+                          / ((demap_ms + decode_ms) * 1e-3) / 1e9, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+    }
+    other = "two" if fused else "fused"
+    if other in runs:
+        o_el, o_mms, o_dms, o_sms = runs[other]
+        alt = {
+            "tail": ("two launches: demapping kernel, then the decoder" if other == "two"
+                     else "one launch: max-log demapping fused into the decoder's frame load"),
+            "value": round(args.steps * F * code.k / o_el / 1e9, 3),
+            "ms_per_step": round(o_el / args.steps * 1e3, 3),
+            "frame_latency_ms": pct(o_sms),
+            "kernels_ms": ({"demap": round(o_mms, 4), "decode": round(o_dms, 4)}
+                           if other == "two" else {"demap+decode (fused)": round(o_dms, 4)}),
+            "roofline": decoder_roofline(other, o_dms),
+            "outputs_equal": tails_equal,
+        }
+        if other == "two":
+            alt["demap_roofline"] = demapper_roofline(o_mms)
+        line["other_tail"] = alt
+    line["hbm_copy_GBs"] = round(copy_bandwidth(dev), 1)
     progress(leg="c2", value=line["value"], decode_ms=round(decode_ms, 4))
     # Parity: the oracle decodes every frame's GPU LLRs (for the fused tail,
     # those of the demapping kernel, which the fused load reproduces bit for
@@ -600,7 +644,7 @@ def run_single(args, dev) -> tuple[dict, bool]:
         line["cpu_baseline"] = cpu_line(frames * code.k / dt / 1e9,
                                         f"{frames} of the {F} frames (max-log LLRs + layered "
                                         f"min-sum), {dt:.2f} s per run")  # fmt: skip
-    return line, parity["identical"] == parity["frames"]
+    return line, parity["identical"] == parity["frames"] and tails_equal
 
 
 # -----------------------------------------------------------------------------

@@ -1,8 +1,11 @@
 // Bit-LLR demapping on gfx950: three streaming kernels, one symbol per lane.
 //
 //   separable_closed  max-log LLRs of a uniform separable lattice: per axis,
+//                     the cell of x (nearest level and side) from one float64
+//                     comparison, then per bit (x - mid) * coef from the
+//                     cell's tables (ck_closed.h; axes of up to 6 bits), or
 //                     the snapped level against the nearer of the two levels
-//                     just outside its run of equal bits (1 + 2 kb metrics);
+//                     just outside its run of equal bits (wider axes);
 //   separable_levels  the metrics of each axis's levels with the log prior,
 //                     max-log or exact log-sum-exp: unimodal Gray forms
 //                     (compile-time halves) for Gray labels over equally
@@ -953,13 +956,21 @@ __global__ __launch_bounds__(kThreads) void llr_points_kernel(
     }
 }
 
-int grid_for(long long n) {
+// Workgroups of a grid-stride launch over n symbols: per_cu per CU at most.
+// The cell-table closed form streams at close to the copy rate with 8 per
+// CU (fewer trips stage the tables; measured 8 against 16: 79 against 87 us
+// on 12.3 M symbols of 64-QAM); the level-metric forms keep 16 to hide
+// their longer per-symbol latency.
+int grid_for(long long n, int per_cu = 16) {
     ck::DeviceInfo info{};
     if (ck::device_info(&info) != CK_OK) info.num_cus = 256;
     const long long tiles = (n + kThreads - 1) / kThreads;
-    const long long cap = static_cast<long long>(info.num_cus) * 16;
+    const long long cap = static_cast<long long>(info.num_cus) * per_cu;
     return static_cast<int>(tiles < cap ? tiles : cap);
 }
+
+template <int KB, bool CLOSED>
+constexpr int kPerCu = kCellTables<KB, CLOSED> ? 8 : 16;
 
 int32_t check_axis(const ck_axis* a, const char* name, bool closed) {
     CK_REQUIRE(a != nullptr, "%s axis is NULL", name);
@@ -996,13 +1007,13 @@ int32_t launch_separable(const void* x, int64_t n, const ck_axis* re, const ck_a
     CK_REQUIRE(inv_s2 > 0.0 && inv_s2 < 1e300, "inv_s2 must be positive and finite");
     if (n == 0) return CK_OK;
     CK_REQUIRE(x != nullptr && out != nullptr, "NULL symbols or output");
-    const int grid = grid_for(n);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const float2* xs = static_cast<const float2*>(x);
     switch (re->num_bits) {
 #define CK_CASE(KB)                                                                           \
     case KB:                                                                                  \
-        hipLaunchKernelGGL((separable_kernel<KB, CLOSED>()), dim3(grid), dim3(kThreads), 0,   \
+        hipLaunchKernelGGL((separable_kernel<KB, CLOSED>()),                                  \
+                           dim3(grid_for(n, kPerCu<KB, CLOSED>)), dim3(kThreads), 0,          \
                            s, xs, static_cast<long long>(n), *re, *im,                        \
                            static_cast<float>(inv_s2), inv_s2, exact, out, inv_rows,          \
                            static_cast<long long>(row_len), perm,                             \
@@ -1086,11 +1097,12 @@ template <bool CLOSED>
 int32_t launch_segments(const Segments& sg, const ck_axis* re, const ck_axis* im, int exact,
                         hipStream_t s) {
     const long long tiles = sg.tile_end[sg.count - 1];
-    const int grid = grid_for(tiles * kThreads);
     switch (re->num_bits) {
 #define CK_CASE(KB)                                                                        \
     case KB:                                                                               \
-        hipLaunchKernelGGL((segments_kernel<KB, CLOSED>()), dim3(grid), dim3(kThreads), 0, \
+        hipLaunchKernelGGL((segments_kernel<KB, CLOSED>()),                                \
+                           dim3(grid_for(tiles * kThreads, kPerCu<KB, CLOSED>)),           \
+                           dim3(kThreads), 0,                                              \
                            s, sg, *re, *im, exact);                                        \
         break;
         CK_CASE(1) CK_CASE(2) CK_CASE(3) CK_CASE(4) CK_CASE(5) CK_CASE(6) CK_CASE(7) CK_CASE(8)
