@@ -66,9 +66,9 @@ LEGS = ("c1", "c3", "c4", "c5")
 # Counters cannot be read inside this process; --traffic reads them live in
 # child processes under rocprofv3 instead.
 PROFILED_TRAFFIC = {
-    "c2": {"frames": FRAMES, "bytes": 2.120e9, "demap_bytes": 0.393e9,
+    "c2": {"frames": FRAMES, "bytes": 2.116e9, "demap_bytes": 0.3933e9,
            "profile": "profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md"},
-    "c2_fused": {"frames": FRAMES, "bytes": None, "demap_bytes": None,
+    "c2_fused": {"frames": FRAMES, "bytes": 1.828e9, "demap_bytes": None,
                  "profile": "profiles/ldpc_qc_layered_r45-z360_64qam_4096cw.md"},
     "c5": {"frames": 3106, "bytes": 8.413e9, "demap_bytes": 0.3366e9,
            "profile": "profiles/ldpc_sum_product_shaped256_3106cw.md"},
@@ -82,8 +82,9 @@ LIMITER = {
           "traffic is a fifth of the algorithmic bytes",
     "c5": "VALU issue, the phi evaluations first (SQ_INSTS_VALU fills ~65% of SIMD cycles, "
           "73% by SQ_ACTIVE_INST_VALU; 6.5 transcendentals per edge and iteration)",
-    "c2_demap": "VALU issue (~70% of SIMD cycles) and load latency (60% of wave cycles "
-                "waiting); a device copy of the same bytes runs at 5.3 TB/s",
+    "c2_demap": "HBM: waves wait on memory 71% of their cycles while 102 VALU instructions "
+                "per symbol issue 46% of the SIMD cycles; a device copy of the same bytes "
+                "runs at 5.3 TB/s",
     "c5_demap": "VALU issue (~70% of SIMD cycles, 1017 instructions per symbol: float64 level "
                 "metrics, one exp2 per level)",
 }  # fmt: skip
