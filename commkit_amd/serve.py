@@ -29,6 +29,7 @@ No counterpart in the reference, which decodes one call at a time
 from __future__ import annotations
 
 import ctypes
+import gc
 import time
 from dataclasses import dataclass, field
 from typing import Any, Callable, Sequence
@@ -362,8 +363,25 @@ def drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None],
     after each turn's fills and before its :meth:`Server.step` (a fill that
     only queues its work, as :class:`~commkit_amd.mapping.demap.
     SegmentsDemapper.add` does, launches it there: the step's launch waits
-    for the work enqueued on the caller's stream).  Returns per-frame
-    latencies (completion seen minus arrival)."""
+    for the work enqueued on the caller's stream).  Python's cyclic garbage
+    collector is paused while the loop runs (the loop makes no reference
+    cycles; a full collection over a large heap stalls admission and polling
+    for milliseconds) and restored after.  Returns per-frame latencies
+    (completion seen minus arrival)."""
+    gc_was_enabled = gc.isenabled()
+    gc.disable()
+    try:
+        return _drive(server, schedule, fill, clock, sleep, idle_s, on_done, admit_slice_s,
+                      before_step)
+    finally:
+        if gc_was_enabled:
+            gc.enable()
+
+
+def _drive(server: Server, schedule: Schedule, fill: Callable[[Slot, int], None],
+           clock: Callable[[], float], sleep: Callable[[float], None] | None, idle_s: float,
+           on_done: Callable[[Slot, int], None] | None, admit_slice_s: float,
+           before_step: Callable[[], None] | None) -> LoadResult:
     n = len(schedule)
     t_arr = schedule.t
     t_done = np.full(n, np.nan)
