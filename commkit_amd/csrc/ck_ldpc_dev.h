@@ -848,6 +848,14 @@ __device__ __forceinline__ float phi2_fast(float y) {
 // every lane gathers the row's phi values of each slot (lane p holds edge
 // tP + p) and adds them in edge order, so all lanes hold the one-lane sum
 // and every tile decodes bit for bit alike.  Padding edges add +0.
+//
+// Signs as in qc_ms_row_n: no Q is -0.0, so Q < 0 is Q's sign bit; the
+// row's sign parity is the XOR of the valid Q's bit patterns (bit 31), and
+// it is folded into the ln 2 that scales each new magnitude, so a new
+// message is that product with Q's sign bit XORed in (one bitop3).  The
+// messages of the lane's edges sit P Z floats apart from its first edge's
+// slot s0 + p; a padding slot (e >= dc, possibly past the last slot) reads
+// slot s0 instead.
 template <int P, int NT, bool UPDATE_APP>
 __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __restrict__ Rm,
                                             int s0, int dc, int r, int p, int Z, bool first) {
@@ -858,23 +866,25 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
     float* ar = a + r;
     const int z4 = neg_col_bytes(Z);
     const unsigned rr = row_rank(r);
+    float* rm = Rm + (static_cast<size_t>(s0 + p) * Z + r);  // edge p's message
+    const int step = P * Z;
     float rold[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
         const bool valid = (t < NT - S) || (e < dc);
         addr[t] = &lds_at(ar, qc_off(tab[s0 + t * P + p], rr, z4));
-        rold[t] = first ? 0.0f : Rm[static_cast<size_t>(valid ? s0 + e : s0) * Z + r];
+        rold[t] = first ? 0.0f : rm[valid ? t * step : -p * Z];  // padding: slot s0
     }
     float total = 0.0f;
-    unsigned neg = 0u;
+    unsigned neg = 0u;  // bit 31: parity of the valid Q's signs
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
         const bool valid = (t < NT - S) || (e < dc);
         q[t] = *addr[t] - rold[t];
         ph[t] = valid ? phi2_fast(fabsf(q[t]) * kLog2eF) : 0.0f;
-        neg ^= (valid && q[t] < 0.0f) ? 1u : 0u;
+        neg ^= valid ? __float_as_uint(q[t]) : 0u;
     }
     if constexpr (P == 1) {
 #pragma unroll
@@ -902,14 +912,18 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
         for (int off = 1; off < P; off <<= 1)
             neg ^= static_cast<unsigned>(lane_xor(static_cast<int>(neg), off));
     }
+    // -ln 2 for an odd row parity: the product carries the parity's sign.
+    float ln2s = (neg & 0x80000000u) ? -kLn2F : kLn2F;
+    asm volatile("" : "+v"(ln2s));  // one select per row, not per edge
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
         if ((t < NT - S) || (e < dc)) {
-            const float mag = phi2_fast(total - ph[t]) * kLn2F;
-            const float rn = (neg ^ ((q[t] < 0.0f) ? 1u : 0u)) ? -mag : mag;
+            const float mag = phi2_fast(total - ph[t]) * ln2s;
+            const float rn = __uint_as_float(__float_as_uint(mag) ^
+                                             (__float_as_uint(q[t]) & 0x80000000u));
             if (UPDATE_APP) *addr[t] = q[t] + rn;
-            Rm[static_cast<size_t>(s0 + e) * Z + r] = rn;
+            rm[t * step] = rn;
         }
     }
 }
