@@ -169,17 +169,30 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
         store_frame(c, a, l.frame, l.u, Z * P, iters, ok, app_out, bits_out, iters_out, valid_out);
 }
 
+// G groups of Z P lanes per frame: in the check phase group g updates block
+// rows g, g + G, ... (flooding rows are independent within an iteration:
+// each reads the previous iteration's APP and its own state), and the
+// variable phase and the frame's store spread over all G Z P lanes.  The
+// outputs are the same for every G.
 template <bool SP, int P>
 __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
-    unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, Params prm) {
+    unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, int G,
+    Params prm) {
     extern __shared__ float lds[];
     __shared__ int s_flag[kMaxFrames];
     __shared__ int2 s_tab[kTabEntries];
     const int N = c.num_variables;
     const int Z = c.lifting;
-    const QcLane l = qc_lane<P>(Z, F, R);
+    // Lanes of (frame f, group g) as those of a virtual frame f G + g.
+    QcLane l = qc_lane<P>(Z, F * G, static_cast<long long>(F) * G * gridDim.x);
+    const int g = l.f % G;
+    l.f = l.f / G;
+    l.frame = static_cast<long long>(blockIdx.x) * F + l.f;
+    l.live = (l.f < F) && (l.frame < R);
+    const int lanes = G * Z * P;      // lanes of one frame
+    const int ug = g * Z * P + l.u;  // this lane among them
     float* a = lds + static_cast<size_t>(l.f < F ? l.f : 0) * N;
     stage_slot_table(c, s_tab);
     load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
@@ -210,7 +223,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         const bool first = it == 0;
         // Check phase: every row reads the previous iteration's APP.
         if (!done) {
-            for (int br = 0; br < c.base_rows; ++br) {
+            for (int br = g; br < c.base_rows; br += G) {
                 if (SP) {
                     if (l.p == 0) qc_sp_row<1, false>(c, s_tab, a, Rm, br, l.r, 0, first);
                 } else {
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         __syncthreads();
         // Variable phase: APP = L + the new R of every check, ascending check.
         if (!done) {
-            for (int v = l.u; v < N; v += Z * P) {
+            for (int v = ug; v < N; v += lanes) {
                 const int j = v / Z;
                 const int cc = v - j * Z;
                 float sum = v < c.n ? L[v] + 0.0f : 0.0f;  // -0.0 as +0.0, as at the load
@@ -251,7 +264,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     }
     __syncthreads();
     if (l.live)
-        store_frame(c, a, l.frame, l.u, Z * P, iters, ok, app_out, bits_out, iters_out, valid_out);
+        store_frame(c, a, l.frame, ug, lanes, iters, ok, app_out, bits_out, iters_out, valid_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -600,6 +613,7 @@ struct Plan {
     bool state_lds;    // QC layered min-sum
     int short_rows;    // QC layered min-sum, rows of at most kShortRow edges: the
                        // SHORT variant (1: 168 VGPRs, 2: 128 VGPRs), else 0
+    int G;             // QC flooding: block rows updated at once per frame
     int threads;
     size_t lds_bytes;
     size_t ws_per_frame;  // bytes
@@ -736,7 +750,10 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, P
         p.F = F;
         p.P = P;
         p.state_lds = state_lds;
-        p.threads = static_cast<int>(round_up(static_cast<size_t>(F) * Z * P, 64));
+        p.G = 1;
+        if (p.flood)  // as many of a frame's block rows at once as the workgroup holds
+            p.G = std::max(1, std::min(c->base_rows, kMaxQcThreads / (F * Z * P)));
+        p.threads = static_cast<int>(round_up(static_cast<size_t>(F) * p.G * Z * P, 64));
         p.lds_bytes = qc_lds(F, N, Z, c->num_checks, state_lds).end * 4;
         p.short_rows = 0;
         if (!p.sp && !p.flood && P <= 2 && c->max_row_degree <= kShortRow) {
@@ -793,7 +810,7 @@ void launch_qc_flooding(const Plan& p, const ck_ldpc_code& c, const float* llr, 
     allow_lds(kernel, p.lds_bytes);
     const long long groups = (R + p.F - 1) / p.F;
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(groups)), dim3(p.threads), p.lds_bytes,
-                       s, c, llr, app, bits, iters, valid, ws, R, p.F, prm);
+                       s, c, llr, app, bits, iters, valid, ws, R, p.F, p.G, prm);
 }
 
 template <bool SP, bool FLOOD, bool APP_LDS, int MAXD>
