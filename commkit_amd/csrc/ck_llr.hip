@@ -802,6 +802,160 @@ __device__ __forceinline__ double point_diff(const double2* pts, const double* l
 
 constexpr int kPointsInLds = 1024;
 
+// Constellations of up to 2^kOrderedBits points are staged in label order
+// (point m has label m, when the labels are a permutation of 0 .. M-1, as
+// every constellation's are) and the max-log point loop is unrolled, so the
+// half of each bit a point belongs to is a compile-time constant.
+constexpr int kOrderedBits = 8;
+
+// The LLRs of one symbol (xr, xi) from the metrics of all M = 2^K points,
+// into row[b].  ORDERED: the points are in label order and the loops
+// unrolled.  Max-log locates each half's maximum in float64 (the maxima
+// are the answer) and returns their factored float64 difference; ordered,
+// a half's maximum is tracked as one float64 max of a key, the metric with
+// its lowest K mantissa bits replaced by the point's label (metrics within
+// 2^-(52-K) relative of each other may swap, and either gives the same LLR
+// to that precision), the label of the maximum read back from the key.
+// Exact takes the peak point (float32 metrics) as reference: every term
+// 2^w, w = M_m - M_pk in log2 units (float64, then exp2_split), goes to one
+// of the two float64 half sums of each bit; a point's label is the same in
+// every lane, read on the scalar unit, and each half sum takes one FMA with
+// a scalar-selected weight of 1 or 0 (exact: S + 1 e = S + e, S + 0 e = S).
+// (Its point loops stay rolled: unrolled over 128 points the exact form ran
+// 1.7 times slower.)
+template <int K, bool ORDERED, bool STAGED>
+__device__ __forceinline__ void points_symbol(const double2* pts, const double* logp,
+                                              const double* logp2, const int* lab_tab,
+                                              float2 v, float inv_s2f, double inv_s2,
+                                              int exact, float* row) {
+    constexpr int M = 1 << K;
+    const double s2l = inv_s2 * kLog2eD;
+    const double xr = v.x;
+    const double xi = v.y;
+    // The metric of point m in log2 units relative to a reference metric
+    // (unstaged tables carry no log2 copy of the prior).
+    const auto metric2 = [&](int m, double ref) {
+        const double2 p = pts[m];
+        const double dr = xr - p.x;
+        const double di = xi - p.y;
+        const double lp = STAGED ? logp2[m] : logp[m] * kLog2eD;
+        return fma(-s2l, fma(dr, dr, di * di), lp - ref);
+    };
+    if (!exact) {
+        int a0[K], a1[K];
+        if constexpr (ORDERED) {
+            double k0[K], k1[K];
+#pragma unroll
+            for (int b = 0; b < K; ++b) {
+                k0[b] = -ck::kInf;
+                k1[b] = -ck::kInf;
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const double2 p = pts[m];
+                const double dr = xr - p.x;
+                const double di = xi - p.y;
+                const double met = -(dr * dr + di * di) * inv_s2 + logp[m];
+                const double key = __longlong_as_double(
+                    (__double_as_longlong(met) & ~static_cast<long long>(M - 1)) | m);
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    if ((m >> (K - 1 - b)) & 1)
+                        k1[b] = fmax(k1[b], key);
+                    else
+                        k0[b] = fmax(k0[b], key);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < K; ++b) {  // NaN symbols: keys stay -inf, label 0
+                a0[b] = static_cast<int>(__double_as_longlong(k0[b]) & (M - 1));
+                a1[b] = static_cast<int>(__double_as_longlong(k1[b]) & (M - 1));
+            }
+        } else {
+            double p0[K], p1[K];
+#pragma unroll
+            for (int b = 0; b < K; ++b) {
+                p0[b] = -1.0e300;
+                p1[b] = -1.0e300;
+                a0[b] = 0;
+                a1[b] = 0;
+            }
+            for (int m = 0; m < M; ++m) {
+                const double2 p = pts[m];
+                const double dr = xr - p.x;
+                const double di = xi - p.y;
+                const double met = -(dr * dr + di * di) * inv_s2 + logp[m];
+                const int lab = lab_tab[m];
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    if ((lab >> (K - 1 - b)) & 1) {
+                        if (met > p1[b]) { p1[b] = met; a1[b] = m; }
+                    } else {
+                        if (met > p0[b]) { p0[b] = met; a0[b] = m; }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < K; ++b)
+            row[b] = static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2, a0[b], a1[b]));
+        return;
+    }
+    float best = -ck::kInf;
+    int pk = 0;
+    for (int m = 0; m < M; ++m) {
+        const double2 p = pts[m];
+        const float dr = v.x - static_cast<float>(p.x);
+        const float di = v.y - static_cast<float>(p.y);
+        const float met = -(dr * dr + di * di) * inv_s2f + static_cast<float>(logp[m]);
+        if (met > best) {
+            best = met;
+            pk = m;
+        }
+    }
+    const double ref = metric2(pk, 0.0);
+    double S0[K], S1[K];
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+        S0[b] = 0.0;
+        S1[b] = 0.0;
+    }
+    for (int m = 0; m < M; ++m) {
+        const double e = exp2_split(metric2(m, ref));
+        const int lab = __builtin_amdgcn_readfirstlane(lab_tab[m]);
+#pragma unroll
+        for (int b = 0; b < K; ++b) {
+            const bool one = (lab >> (K - 1 - b)) & 1;
+            S1[b] = fma(one ? 1.0 : 0.0, e, S1[b]);
+            S0[b] = fma(one ? 0.0 : 1.0, e, S0[b]);
+        }
+    }
+    const int lpk = lab_tab[pk];
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+        const int own = (lpk >> (K - 1 - b)) & 1;  // the peak's half: sum >= 1
+        const double s_own = own ? S1[b] : S0[b];
+        const double s_oth = own ? S0[b] : S1[b];
+        float r;  // log(S_own / S_oth)
+        if (s_oth >= kFarHalf) {
+            r = log_ratio(s_own, s_oth);
+        } else {
+            // The other half lies more than 1000 octaves under the peak:
+            // summed again relative to its own best.
+            double mo = -1.0e300;
+            for (int m = 0; m < M; ++m)
+                if (((lab_tab[m] >> (K - 1 - b)) & 1) != own) mo = fmax(mo, metric2(m, ref));
+            double u = 0.0;
+            for (int m = 0; m < M; ++m)
+                if (((lab_tab[m] >> (K - 1 - b)) & 1) != own)
+                    u += exp2_split(metric2(m, ref) - mo);
+            r = static_cast<float>(-mo * 0.6931471805599453 +
+                                   static_cast<double>(log_of(s_own) - log_of(u)));
+        }
+        row[b] = own ? -r : r;
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(kThreads) void llr_points_kernel(
     const float2* __restrict__ x, long long n, const double2* __restrict__ g_pts,
@@ -813,20 +967,43 @@ __global__ __launch_bounds__(kThreads) void llr_points_kernel(
     // points are staged in LDS, decided at compile time so that the
     // compiler reads them as LDS, not through generic pointers.
     constexpr bool staged = (1 << K) <= kPointsInLds;
+    static_assert(!(K <= kOrderedBits) || staged, "ordered tables are staged");
     const double2* pts = g_pts;
     const double* logp = g_logp;
-    const double* logp2 = nullptr;  // log prior in log2 units (staged only)
+    const double* logp2 = nullptr;
     const int* lab_tab = g_lab;
+    bool ordered = false;
     if constexpr (staged) {
         double2* s_pts = reinterpret_cast<double2*>(dyn);
         double* s_logp = reinterpret_cast<double*>(s_pts + M);
         double* s_logp2 = s_logp + M;
         int* s_lab = reinterpret_cast<int*>(s_logp2 + M);
-        for (int i = threadIdx.x; i < M; i += kThreads) {
-            s_pts[i] = g_pts[i];
-            s_logp[i] = g_logp[i];
-            s_logp2[i] = g_logp[i] * kLog2eD;
-            s_lab[i] = g_lab[i];
+        if constexpr (K <= kOrderedBits) {
+            // Point i at position lab[i]; the staging is ordered when every
+            // position m then holds label m (the labels are a permutation).
+            for (int i = threadIdx.x; i < M; i += kThreads) s_lab[i] = -1;
+            __syncthreads();
+            for (int i = threadIdx.x; i < M; i += kThreads) {
+                const int L = g_lab[i];
+                if (L >= 0 && L < M) {
+                    s_pts[L] = g_pts[i];
+                    s_logp[L] = g_logp[i];
+                    s_logp2[L] = g_logp[i] * kLog2eD;
+                    s_lab[L] = L;
+                }
+            }
+            __syncthreads();
+            bool ok = true;
+            for (int i = threadIdx.x; i < M; i += kThreads) ok = ok && s_lab[i] == i;
+            ordered = __syncthreads_and(ok ? 1 : 0) != 0;
+        }
+        if (!ordered) {
+            for (int i = threadIdx.x; i < M; i += kThreads) {
+                s_pts[i] = g_pts[i];
+                s_logp[i] = g_logp[i];
+                s_logp2[i] = g_logp[i] * kLog2eD;
+                s_lab[i] = g_lab[i];
+            }
         }
         pts = s_pts;
         logp = s_logp;
@@ -834,119 +1011,22 @@ __global__ __launch_bounds__(kThreads) void llr_points_kernel(
         lab_tab = s_lab;
     }
     __syncthreads();
-    const double s2l = inv_s2 * kLog2eD;
-    // The metric of point m in log2 units relative to a reference metric.
-    const auto metric2 = [&](double xr, double xi, int m, double ref) {
-        const double2 p = pts[m];
-        const double dr = xr - p.x;
-        const double di = xi - p.y;
-        double lp;
-        if constexpr (staged)
-            lp = logp2[m];
-        else
-            lp = logp[m] * kLog2eD;
-        return fma(-s2l, fma(dr, dr, di * di), lp - ref);
-    };
     float* row = tile + threadIdx.x * tile_stride(K);
     for (long long base = static_cast<long long>(blockIdx.x) * kThreads; base < n;
          base += static_cast<long long>(gridDim.x) * kThreads) {
         const long long i = base + threadIdx.x;
         if (i < n) {
             const float2 v = x[i];
-            const double xr = v.x;
-            const double xi = v.y;
-            int a0[K], a1[K];
-            // Maxima of each half: located in float64 for max-log (they are
-            // the answer), in float32 for exact (only a reference point).
-            if (!exact) {
-                double p0[K], p1[K];
-#pragma unroll
-                for (int b = 0; b < K; ++b) {
-                    p0[b] = -1.0e300;
-                    p1[b] = -1.0e300;
-                    a0[b] = 0;
-                    a1[b] = 0;
-                }
-                for (int m = 0; m < M; ++m) {
-                    const double2 p = pts[m];
-                    const double dr = xr - p.x;
-                    const double di = xi - p.y;
-                    const double met = -(dr * dr + di * di) * inv_s2 + logp[m];
-                    const int lab = lab_tab[m];
-#pragma unroll
-                    for (int b = 0; b < K; ++b) {
-                        if ((lab >> (K - 1 - b)) & 1) {
-                            if (met > p1[b]) { p1[b] = met; a1[b] = m; }
-                        } else {
-                            if (met > p0[b]) { p0[b] = met; a0[b] = m; }
-                        }
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < K; ++b)
-                    row[b] = static_cast<float>(point_diff(pts, logp, xr, xi, inv_s2, a0[b], a1[b]));
+            if constexpr (K <= kOrderedBits) {
+                if (ordered)
+                    points_symbol<K, true, staged>(pts, logp, logp2, lab_tab, v, inv_s2f, inv_s2,
+                                                   exact, row);
+                else
+                    points_symbol<K, false, staged>(pts, logp, logp2, lab_tab, v, inv_s2f,
+                                                    inv_s2, exact, row);
             } else {
-                // The peak point (float32 metrics) is the reference: every
-                // term 2^w, w = M_m - M_pk in log2 units (float64, then
-                // exp2_split), goes to one of the two float64 half sums of
-                // each bit.  A point's label is the same for every lane, so
-                // the half of each bit is a uniform branch, not a select.
-                float best = -ck::kInf;
-                int pk = 0;
-                for (int m = 0; m < M; ++m) {
-                    const double2 p = pts[m];
-                    const float dr = v.x - static_cast<float>(p.x);
-                    const float di = v.y - static_cast<float>(p.y);
-                    const float met = -(dr * dr + di * di) * inv_s2f + static_cast<float>(logp[m]);
-                    if (met > best) {
-                        best = met;
-                        pk = m;
-                    }
-                }
-                const double ref = metric2(xr, xi, pk, 0.0);
-                double S0[K], S1[K];
-#pragma unroll
-                for (int b = 0; b < K; ++b) {
-                    S0[b] = 0.0;
-                    S1[b] = 0.0;
-                }
-                for (int m = 0; m < M; ++m) {
-                    const double e = exp2_split(metric2(xr, xi, m, ref));
-                    const int lab = __builtin_amdgcn_readfirstlane(lab_tab[m]);
-#pragma unroll
-                    for (int b = 0; b < K; ++b) {
-                        // 0/1 weights chosen on the scalar unit: one FMA per
-                        // half (S + 1 e = S + e, S + 0 e = S exactly).
-                        const bool one = (lab >> (K - 1 - b)) & 1;
-                        S1[b] = fma(one ? 1.0 : 0.0, e, S1[b]);
-                        S0[b] = fma(one ? 0.0 : 1.0, e, S0[b]);
-                    }
-                }
-                const int lpk = lab_tab[pk];
-#pragma unroll
-                for (int b = 0; b < K; ++b) {
-                    const int own = (lpk >> (K - 1 - b)) & 1;  // the peak's half: sum >= 1
-                    const double s_own = own ? S1[b] : S0[b];
-                    const double s_oth = own ? S0[b] : S1[b];
-                    float r;  // log(S_own / S_oth)
-                    if (s_oth >= kFarHalf) {
-                        r = log_ratio(s_own, s_oth);
-                    } else {
-                        // The other half lies more than 1000 octaves under
-                        // the peak: summed again relative to its own best.
-                        double mo = -1.0e300;
-                        for (int m = 0; m < M; ++m)
-                            if (((lab_tab[m] >> (K - 1 - b)) & 1) != own)
-                                mo = fmax(mo, metric2(xr, xi, m, ref));
-                        double u = 0.0;
-                        for (int m = 0; m < M; ++m)
-                            if (((lab_tab[m] >> (K - 1 - b)) & 1) != own)
-                                u += exp2_split(metric2(xr, xi, m, ref) - mo);
-                        r = static_cast<float>(-mo * 0.6931471805599453 +
-                                               static_cast<double>(log_of(s_own) - log_of(u)));
-                    }
-                    row[b] = own ? -r : r;
-                }
+                points_symbol<K, false, staged>(pts, logp, logp2, lab_tab, v, inv_s2f, inv_s2,
+                                                exact, row);
             }
         }
         __syncthreads();
