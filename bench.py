@@ -80,8 +80,9 @@ LIMITER = {
     "c2": "VALU issue and barrier/LDS waits (SQ_INSTS_VALU fills ~56% of SIMD cycles, "
           "SQ_WAIT_ANY 45% of wave cycles); APP and check state live in LDS, so HBM "
           "traffic is a fifth of the algorithmic bytes",
-    "c5": "VALU issue, the phi evaluations first (SQ_INSTS_VALU fills ~65% of SIMD cycles, "
-          "73% by SQ_ACTIVE_INST_VALU; 6.5 transcendentals per edge and iteration)",
+    "c5": "VALU issue, the phi evaluations first (SQ_INSTS_VALU fills ~61% of SIMD cycles, "
+          "69% by SQ_ACTIVE_INST_VALU; 60 lane instructions per edge and iteration, 6.5 of "
+          "them transcendental), one workgroup per CU (a frame's 86 KB APP)",
     "c2_demap": "HBM: waves wait on memory 71% of their cycles while 102 VALU instructions "
                 "per symbol issue 46% of the SIMD cycles; a device copy of the same bytes "
                 "runs at 5.3 TB/s",
