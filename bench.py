@@ -1163,15 +1163,23 @@ def run_serving(args, dev) -> tuple[dict, bool]:
         for d in seg_dm.values():
             d.flush()
 
+    # Device addresses of each pool's symbols and noise, for the fills.
+    pool_ptrs = {key: [(rx.data_ptr(), inv.data_ptr(), rx.shape[1]) for rx, inv, _ in per_bin]
+                 for key, per_bin in pools.items()}  # fmt: skip
+
     def make_fill(sched: Schedule):
+        stream_pairs = [pairs[int(s)] for s in sched.stream]
+        snr_bins = sched.snr_bin.tolist()
+
         def fill(slot, i):
-            ci, order = pairs[int(sched.stream[i])]
-            rx, inv, _ = pools[(ci, order)][int(sched.snr_bin[i])]
+            ci, order = stream_pairs[i]
             f = slot.frames
             if seg_dm:
+                rx, inv, _ = pools[(ci, order)][snr_bins[i]]
                 seg_dm[order].add(rx[:f], slot.llrs, inv_s2=inv[:f])
             else:
-                rows_dm[order](rx[:f], inv[:f], slot.llrs)
+                x_ptr, inv_ptr, S = pool_ptrs[(ci, order)][snr_bins[i]]
+                rows_dm[order].raw(x_ptr, f, S, inv_ptr, slot.llrs_ptr)
         return fill
 
     def codes_of(sched: Schedule) -> Schedule:
@@ -1331,6 +1339,7 @@ def main() -> int:
         print(json.dumps({"metric": METRIC, "n_gpus": 1, **out}), flush=True)
         return 0 if ok else 1
     line, ok = run_single(args, dev)
+    torch.cuda.empty_cache()
     if args.workload == "all":
         legs = [x for x in args.legs.split(",") if x]
         runners = {"c1": ("c1_plain", run_plain), "c3": ("c3_mixed", run_mixed),

@@ -339,6 +339,7 @@ class RowsDemapper:
         self.k = c.bits_per_symbol
         self._lib = _native.lib()
         self._dtypes = (torch.complex64, torch.float64, torch.float32)
+        self._axes = (ctypes.byref(self._re), ctypes.byref(self._im))
 
     def __call__(self, x: Any, inv_s2: Any, out: Any) -> Any:
         R, S = int(x.shape[0]), int(x.shape[1])
@@ -355,6 +356,17 @@ class RowsDemapper:
                 x.data_ptr(), R, S, ctypes.byref(self._re), ctypes.byref(self._im),
                 inv_s2.data_ptr(), self._closed, self._exact, out.data_ptr(), self._stream))  # fmt: skip
         return out
+
+    def raw(self, x: int, rows: int, row_len: int, inv_s2: int, out: int) -> None:
+        """The call on device addresses, unchecked, for request loops that
+        keep their buffers' addresses (``Slot.llrs_ptr`` of the serving
+        engine): ``x`` ``(rows, row_len)`` complex64, ``inv_s2`` ``(rows,)``
+        float64 and ``out`` ``(rows, row_len * k)`` float32, contiguous, on
+        the demapper's device."""
+        if rows * row_len:
+            _native.check(self._lib.ck_llr_separable_rows(
+                x, rows, row_len, self._axes[0], self._axes[1], inv_s2, self._closed,
+                self._exact, out, self._stream))  # fmt: skip
 
 
 class CkLlrSegment(ctypes.Structure):

@@ -64,20 +64,42 @@ class CkServeSlot(ctypes.Structure):
                 ("first", ctypes.c_int64)]
 
 
-@dataclass
 class Slot:
     """An admitted request: its code and frame range in that code's ring,
-    and (not in a dry run) device tensor views of its buffers."""
+    the device address of its LLR rows (``llrs_ptr``, 0 in a dry run), and
+    (not in a dry run) device tensor views of its buffers, ``llrs``,
+    ``app``, ``bits``, ``iterations`` and ``valid``, made on first use (a
+    request loop that fills the LLRs through ``llrs_ptr`` and reads results
+    with device work of its own never pays for the views)."""
 
-    tag: int
-    code: int
-    first: int
-    frames: int
-    llrs: Any = None
-    app: Any = None
-    bits: Any = None
-    iterations: Any = None
-    valid: Any = None
+    __slots__ = ("tag", "code", "first", "frames", "llrs_ptr", "_ring", "_views")
+    _NAMES = ("llrs", "app", "bits", "iterations", "valid")
+
+    def __init__(self, tag: int, code: int, first: int, frames: int, llrs_ptr: int = 0,
+                 ring: dict[str, Any] | None = None) -> None:
+        self.tag, self.code, self.first, self.frames = tag, code, first, frames
+        self.llrs_ptr = llrs_ptr
+        self._ring = ring
+        self._views: dict[str, Any] = {}
+
+    def _view(self, name: str) -> Any:
+        if self._ring is None:
+            return None
+        v = self._views.get(name)
+        if v is None:
+            v = self._ring[name][self.first : self.first + self.frames]
+            self._views[name] = v
+        return v
+
+    llrs = property(lambda self: self._view("llrs"))
+    app = property(lambda self: self._view("app"))
+    bits = property(lambda self: self._view("bits"))
+    iterations = property(lambda self: self._view("iterations"))
+    valid = property(lambda self: self._view("valid"))
+
+    def __repr__(self) -> str:
+        return (f"Slot(tag={self.tag}, code={self.code}, first={self.first}, "
+                f"frames={self.frames})")
 
 
 def frame_bytes(code: LDPCCode) -> int:
@@ -205,12 +227,9 @@ class Server:
         if st == CK_EBUSY:
             return None
         _native.check(st)
-        s = Slot(int(tag), int(code), int(where.first), int(frames))
-        if not self.dry_run:
-            t = self.ring_tensors[code]
-            a, b = s.first, s.first + s.frames
-            s.llrs, s.app, s.bits = t["llrs"][a:b], t["app"][a:b], t["bits"][a:b]
-            s.iterations, s.valid = t["iterations"][a:b], t["valid"][a:b]
+        s = Slot(int(tag), int(code), int(where.first), int(frames),
+                 0 if self.dry_run else int(where.llrs or 0),
+                 None if self.dry_run else self.ring_tensors[code])  # fmt: skip
         self._slots[s.tag] = s
         return s
 
