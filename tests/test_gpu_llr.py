@@ -416,3 +416,94 @@ def test_segments_equal_per_request_calls(gpu, make, method):
         assert torch.equal(got, want), f"request {j}"
     with pytest.raises(ValueError, match="exactly one"):
         dm.add(x, out, perm=perm)
+
+
+def _maxlog_factored(x: np.ndarray, points: np.ndarray, bit_labels: np.ndarray,
+                     noise_var: float) -> np.ndarray:
+    """Max-log LLRs of the definition with each half's best point located by
+    float64 metrics and the difference taken in factored form,
+    ``((pa - pc) (2x - pa - pc)).real-and-imag / sigma^2``: free of the
+    cancellation of a difference of two large float64 maxima when x sits on
+    a decision boundary (uniform prior)."""
+    x = np.asarray(x).ravel().astype(np.complex128)
+    pts = np.asarray(points, dtype=np.complex128)
+    k = bit_labels.shape[1]
+    halves = [(np.where(bit_labels[:, b] == 0)[0], np.where(bit_labels[:, b] == 1)[0])
+              for b in range(k)]  # fmt: skip
+    out = np.empty((x.size, k))
+    for i0 in range(0, x.size, 256):
+        xs = x[i0 : i0 + 256]
+        d2 = (xs.real[:, None] - pts.real[None, :]) ** 2 + (xs.imag[:, None] - pts.imag[None, :]) ** 2
+        for b, (zero, one) in enumerate(halves):
+            pa = pts[zero[np.argmin(d2[:, zero], axis=1)]]
+            pc = pts[one[np.argmin(d2[:, one], axis=1)]]
+            num = ((pa.real - pc.real) * ((xs.real - pa.real) + (xs.real - pc.real)) +
+                   (pa.imag - pc.imag) * ((xs.imag - pa.imag) + (xs.imag - pc.imag)))  # fmt: skip
+            out[i0 : i0 + 256, b] = num / noise_var
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("order", [16, 64, 256, 1024, 4096, 16384])
+def test_closed_form_at_cell_boundaries(gpu, order):
+    """The max-log closed form on symbols at every half-grid point of the
+    axes (each level and each midpoint of neighbouring levels, rounded to
+    float32) and at their float32 neighbours, where one float64 comparison
+    decides the cell, plus noisy symbols; every element within 1e-5
+    relative of the definition.  Up to 4096-QAM (6 bits per axis) the
+    kernels use per-cell tables; 16384-QAM takes the snap-and-compare
+    form."""
+    import torch
+
+    c = Constellation.qam(order)
+    lev = c._axes[0].levels.astype(np.float64)
+    half = np.sort(np.concatenate([lev, 0.5 * (lev[1:] + lev[:-1])])).astype(np.float32)
+    grid = np.unique(np.concatenate([half, np.nextafter(half, np.float32(np.inf)),
+                                     np.nextafter(half, np.float32(-np.inf))]))  # fmt: skip
+    r = np.random.default_rng(order)
+    other = r.uniform(lev[0] * 1.1, lev[-1] * 1.1, grid.size).astype(np.float32)
+    nv = float((lev[1] - lev[0]) ** 2)  # one level step per standard deviation
+    x = np.concatenate([grid + 1j * other, other + 1j * grid,
+                        _noisy(c, 1024, nv, seed=order)]).astype(np.complex64)  # fmt: skip
+    got = compute_llr(torch.from_numpy(x).cuda(), noise_var=nv, constellation=c).cpu().numpy()
+    ref = _maxlog_factored(x, c.points, c.bit_labels, nv)
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-37)
+    rel[np.abs(ref) < 1e-37] = 0.0  # float32 denormal results
+    print(f"{order}-QAM: {x.size} symbols, worst error / bar {rel.max() / 1e-5:.4f}")
+    assert rel.max() <= 1e-5
+    assert np.array_equal(np.sign(got[np.abs(ref) >= 1e-37]), np.sign(ref[np.abs(ref) >= 1e-37]))
+
+
+def test_points_with_repeated_labels(gpu):
+    """ck_llr_points with labels that are not a permutation (two points share
+    a label, one label is absent): the kernel stages the tables as given,
+    not in label order, and its LLRs still follow the definition with those
+    labels (every element within 1e-5 relative, max-log and exact)."""
+    import ctypes
+
+    import torch
+
+    from commkit_amd import _native
+
+    c = Constellation.psk(8)
+    labels = np.array([0, 1, 3, 2, 6, 7, 5, 0], dtype=np.int32)  # 4 absent, 0 twice
+    bits = ((labels[:, None] >> np.arange(2, -1, -1)[None, :]) & 1).astype(np.int8)
+    nv = 0.05
+    x = _noisy(c, 8192, nv, seed=21)
+    dev = torch.device("cuda", 0)
+    xs = torch.from_numpy(x).to(dev)
+    pts = torch.from_numpy(c.points.astype(np.complex128)).to(dev)
+    lp = torch.zeros(8, dtype=torch.float64, device=dev)
+    lab = torch.from_numpy(labels).to(dev)
+    lib = _native.lib()
+    for method in ("maxlog", "exact"):
+        out = torch.empty((x.size, 3), dtype=torch.float32, device=dev)
+        _native.check(lib.ck_llr_points(xs.data_ptr(), x.size, pts.data_ptr(), lp.data_ptr(),
+                                        lab.data_ptr(), 8, 3, ctypes.c_double(1.0 / nv),
+                                        int(method == "exact"), out.data_ptr(),
+                                        _native.stream_of(dev)))  # fmt: skip
+        got = out.cpu().numpy().reshape(-1)
+        ref = llr_fp64(x, c.points, bits, None, nv, method)
+        ok, under = llr_tolerance_ok(got, ref, floor=FLOOR[method])
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), max(FLOOR[method], 1e-30))
+        print(f"repeated labels {method}: worst error / bar {rel.max() / 1e-5:.3f}")
+        assert ok
