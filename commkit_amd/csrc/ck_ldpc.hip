@@ -173,14 +173,17 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
 // rows g, g + G, ... (flooding rows are independent within an iteration:
 // each reads the previous iteration's APP and its own state), and the
 // variable phase and the frame's store spread over all G Z P lanes.  The
-// outputs are the same for every G.
+// outputs are the same for every G.  small: the frames' messages (check
+// states or sum-product messages), their channel LLRs and the column tables
+// live in LDS (flood_lds) instead of HBM and the code's tables, so the
+// variable phase's chain of table, state and LLR reads runs from LDS.
 template <bool SP, int P>
 __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, int G,
-    Params prm) {
-    extern __shared__ float lds[];
+    int small, Params prm) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];  // 16-byte check states
     __shared__ int s_flag[kMaxFrames];
     __shared__ int2 s_tab[kTabEntries];
     const int N = c.num_variables;
@@ -201,9 +204,44 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     unsigned* bits = reinterpret_cast<unsigned*>(lds + qc_lds(F, N, Z, c.num_checks, false).bits);
     const bool lane_ok = l.f < F;
     const size_t fr = static_cast<size_t>(l.live ? l.frame : 0);
-    CheckState* st_base = SP ? nullptr : static_cast<CheckState*>(ws) + fr * c.num_checks;
-    float* Rm = SP ? static_cast<float*>(ws) + fr * c.num_edges : nullptr;
-    const float* L = llr + fr * c.n;
+    const int nb = N / Z;
+    const int fl = l.f < F ? l.f : 0;
+    CheckState* st_base;
+    float* Rm;
+    const float* L;  // the frame's channel LLRs; LDS: N of them, -0.0 as +0.0 and zero-padded
+    const int* col_ptr = c.qc_col_ptr;
+    const int* col_slot = c.qc_col_slot;
+    const int* slot_shift = c.qc_slot_shift;
+    const int* slot_row = c.qc_slot_row;
+    const int* slot_pos = c.qc_slot_pos;
+    if (small) {
+        const FloodLds fo = flood_lds(F, N, Z, c.num_checks, c.num_edges, nb, c.num_slots, SP);
+        float* chan = lds + fo.chan;
+        for (int i = threadIdx.x; i < F * N; i += blockDim.x) chan[i] = lds[i];  // the loaded APPs
+        int* tabs = reinterpret_cast<int*>(lds + fo.tabs);
+        const int S = c.num_slots;
+        for (int i = threadIdx.x; i <= nb; i += blockDim.x) tabs[i] = c.qc_col_ptr[i];
+        for (int i = threadIdx.x; i < S; i += blockDim.x) {
+            tabs[nb + 1 + i] = c.qc_col_slot[i];
+            tabs[nb + 1 + S + i] = c.qc_slot_shift[i];
+            tabs[nb + 1 + 2 * S + i] = c.qc_slot_row[i];
+            tabs[nb + 1 + 3 * S + i] = c.qc_slot_pos[i];
+        }
+        col_ptr = tabs;
+        col_slot = tabs + nb + 1;
+        slot_shift = col_slot + S;
+        slot_row = slot_shift + S;
+        slot_pos = slot_row + S;
+        L = chan + static_cast<size_t>(fl) * N;
+        st_base = SP ? nullptr : reinterpret_cast<CheckState*>(lds + fo.msgs) +
+                                     static_cast<size_t>(fl) * c.num_checks;
+        Rm = SP ? lds + fo.msgs + static_cast<size_t>(fl) * align4(c.num_edges) : nullptr;
+        __syncthreads();
+    } else {
+        st_base = SP ? nullptr : static_cast<CheckState*>(ws) + fr * c.num_checks;
+        Rm = SP ? static_cast<float*>(ws) + fr * c.num_edges : nullptr;
+        L = llr + fr * c.n;
+    }
 
     bool done = !l.live;
     bool ok = false;
@@ -240,18 +278,18 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
             for (int v = ug; v < N; v += lanes) {
                 const int j = v / Z;
                 const int cc = v - j * Z;
-                float sum = v < c.n ? L[v] + 0.0f : 0.0f;  // -0.0 as +0.0, as at the load
-                for (int t = c.qc_col_ptr[j]; t < c.qc_col_ptr[j + 1]; ++t) {
-                    const int s = c.qc_col_slot[t];
-                    int rr = cc - c.qc_slot_shift[s];
+                // -0.0 as +0.0, as at the load (the LDS copy is the loaded APP)
+                float sum = small ? L[v] : (v < c.n ? L[v] + 0.0f : 0.0f);
+                for (int t = col_ptr[j]; t < col_ptr[j + 1]; ++t) {
+                    const int s = col_slot[t];
+                    int rr = cc - slot_shift[s];
                     rr += (rr < 0) ? Z : 0;
                     float rv;
                     if (SP) {
                         rv = Rm[static_cast<size_t>(s) * Z + rr];
                     } else {
-                        const CheckState st =
-                            st_base[static_cast<size_t>(c.qc_slot_row[s]) * Z + rr];
-                        rv = state_msg(st, c.qc_slot_pos[s]);
+                        const CheckState st = st_base[static_cast<size_t>(slot_row[s]) * Z + rr];
+                        rv = state_msg(st, slot_pos[s]);
                     }
                     sum = sum + rv;
                 }
@@ -614,6 +652,7 @@ struct Plan {
     int short_rows;    // QC layered min-sum, rows of at most kShortRow edges: the
                        // SHORT variant (1: 168 VGPRs, 2: 128 VGPRs), else 0
     int G;             // QC flooding: block rows updated at once per frame
+    bool small;        // QC flooding: messages, channel LLRs and column tables in LDS
     int threads;
     size_t lds_bytes;
     size_t ws_per_frame;  // bytes
@@ -751,8 +790,13 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, P
         p.P = P;
         p.state_lds = state_lds;
         p.G = 1;
-        if (p.flood)  // as many of a frame's block rows at once as the workgroup holds
+        p.small = false;
+        if (p.flood) {  // as many of a frame's block rows at once as the workgroup holds
             p.G = std::max(1, std::min(c->base_rows, kMaxQcThreads / (F * Z * P)));
+            const size_t small_bytes = flood_lds(F, N, Z, c->num_checks, c->num_edges, N / Z,
+                                                 c->num_slots, p.sp).end * 4;
+            p.small = small_bytes <= lds_max;
+        }
         p.threads = static_cast<int>(round_up(static_cast<size_t>(F) * p.G * Z * P, 64));
         p.lds_bytes = qc_lds(F, N, Z, c->num_checks, state_lds).end * 4;
         p.short_rows = 0;
@@ -762,7 +806,9 @@ int32_t make_plan(const ck_ldpc_code* c, const ck_ldpc_method* m, long long R, P
                              2 * (p.lds_bytes + kStaticLds) <= info.lds_per_block;
             p.short_rows = two ? 2 : 1;
         }
-        if (p.sp)
+        if (p.small)  // flooding with its messages in LDS: no workspace
+            p.ws_per_frame = 0;
+        else if (p.sp)
             p.ws_per_frame = static_cast<size_t>(c->num_edges) * 4;
         else
             p.ws_per_frame = state_lds ? 0 : state_bytes;
@@ -807,10 +853,14 @@ void launch_qc_flooding(const Plan& p, const ck_ldpc_code& c, const float* llr, 
                         signed char* bits, int* iters, unsigned char* valid, void* ws,
                         long long R, const Params& prm, hipStream_t s) {
     auto kernel = ldpc_qc_flooding<SP, P>;
-    allow_lds(kernel, p.lds_bytes);
     const long long groups = (R + p.F - 1) / p.F;
-    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(groups)), dim3(p.threads), p.lds_bytes,
-                       s, c, llr, app, bits, iters, valid, ws, R, p.F, p.G, prm);
+    const size_t lds = p.small ? flood_lds(p.F, c.num_variables, c.lifting, c.num_checks,
+                                           c.num_edges, c.num_variables / c.lifting,
+                                           c.num_slots, SP).end * 4
+                               : p.lds_bytes;
+    allow_lds(kernel, lds);
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(groups)), dim3(p.threads), lds, s, c,
+                       llr, app, bits, iters, valid, ws, R, p.F, p.G, p.small ? 1 : 0, prm);
 }
 
 template <bool SP, bool FLOOD, bool APP_LDS, int MAXD>

@@ -63,6 +63,29 @@ __host__ __device__ __forceinline__ QcLds qc_lds(int F, int N, int Z, int m, boo
     return o;
 }
 
+// The flooding kernel's LDS past the QC layout when a frame's messages fit
+// beside it (small codes), in floats: F frames of channel LLRs (N each),
+// F frames of messages (min-sum: a 16-byte check state per row; sum-product:
+// a float per edge), then the column tables (block column starts, then per
+// slot in column order its slot, and per slot its shift, block row and
+// position).  end == 0: not used.
+struct FloodLds {
+    size_t chan;
+    size_t msgs;
+    size_t tabs;
+    size_t end;
+};
+__host__ __device__ __forceinline__ FloodLds flood_lds(int F, int N, int Z, int m, int E,
+                                                      int nb, int slots, bool sp) {
+    FloodLds o;
+    o.chan = qc_lds(F, N, Z, m, false).end;
+    o.msgs = align4(o.chan + static_cast<size_t>(F) * N);
+    o.tabs = o.msgs + static_cast<size_t>(F) * (sp ? static_cast<size_t>(align4(E))
+                                                 : static_cast<size_t>(m) * 4);
+    o.end = o.tabs + static_cast<size_t>(nb + 1) + 4 * static_cast<size_t>(slots);
+    return o;
+}
+
 __device__ __forceinline__ float state_msg(const CheckState& st, int e) {
     const float mag = (e == st.idx) ? st.m2 : st.m1;  // m1, m2 >= +0
     return __uint_as_float(__float_as_uint(mag) | ((st.signs << (31 - e)) & 0x80000000u));

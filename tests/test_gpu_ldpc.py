@@ -98,9 +98,11 @@ def test_tiles_bit_identical(gpu, name):
         assert_identical(gpu_decode(code, llrs, method, tiles), want, f"{name} {tiles}")
 
 
-@pytest.mark.parametrize("name", ["small", "r1/2-z27", "r2/3-z96"])
+@pytest.mark.parametrize("name", ["small", "r1/2-z27", "r2/3-z96", "r4/5-z360"])
 @pytest.mark.parametrize("early_stop", [False, True])
 def test_min_sum_flooding_bit_exact(gpu, name, early_stop):
+    # The three small codes keep their check states, channel LLRs and column
+    # tables in LDS; r4/5-z360 (n = 18000) keeps them in HBM.
     make, rows = CODES[name]
     code = make()
     llrs = _llrs(code, rows, 1.8, seed=3)
