@@ -176,7 +176,10 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
 // outputs are the same for every G.  small: the frames' messages (check
 // states or sum-product messages), their channel LLRs and the column tables
 // live in LDS (flood_lds) instead of HBM and the code's tables, so the
-// variable phase's chain of table, state and LLR reads runs from LDS.
+// variable phase's chain of table, state and LLR reads runs from LDS.  The
+// syndrome check takes no pass of its own: the check phase reads every
+// row's APPs, and the parity of their signs is the row's syndrome bit for
+// the APP the previous iteration left (the one the stop decision is about).
 template <bool SP, int P>
 __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
@@ -201,7 +204,6 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
     __syncthreads();
 
-    unsigned* bits = reinterpret_cast<unsigned*>(lds + qc_lds(F, N, Z, c.num_checks, false).bits);
     const bool lane_ok = l.f < F;
     const size_t fr = static_cast<size_t>(l.live ? l.frame : 0);
     const int nb = N / Z;
@@ -243,36 +245,48 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         L = llr + fr * c.n;
     }
 
+    // s_flag[f]: some row of frame f has odd hard-decision parity on the APP
+    // its check phase read.
+    if (lane_ok && ug == 0) s_flag[fl] = 0;
+    __syncthreads();
     bool done = !l.live;
     bool ok = false;
     int iters = 0;
     int it = 0;
     for (;;) {
-        // Syndrome checks as in ldpc_qc_layered.
         const bool last = it >= prm.max_iterations;
-        if (prm.early_stop || last) {
-            const bool now = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, !done, F, s_flag);
-            if (!done) {
-                ok = now;
-                if (prm.early_stop) done = now;
-            }
-        }
-        if (last || !__syncthreads_or(!done)) break;
         const bool first = it == 0;
-        // Check phase: every row reads the previous iteration's APP.
+        // Check phase: every row reads the previous iteration's APP, so the
+        // parities of its hard decisions make the syndrome of that APP, the
+        // check before iteration `it` (the APP after `it` iterations).  At
+        // the iteration limit the phase runs for the syndrome alone.
         if (!done) {
             for (int br = g; br < c.base_rows; br += G) {
+                unsigned odd = 0u;
                 if (SP) {
-                    if (l.p == 0) qc_sp_row<1, false>(c, s_tab, a, Rm, br, l.r, 0, first);
+                    if (l.p == 0)
+                        qc_sp_row<1, false, kMaxDegree, true>(c, s_tab, a, Rm, br, l.r, 0, first,
+                                                              &odd);
                 } else {
                     CheckState* sp = st_base + static_cast<size_t>(br) * Z + l.r;
                     const CheckState st = first ? zero_state() : *sp;
-                    const CheckState ns = qc_ms_row<P, false>(c, s_tab, a, st, br, l.r, l.p, prm);
-                    if (l.p == 0) *sp = ns;
+                    const CheckState ns = qc_ms_row<P, false, kMaxDegree / P, true>(
+                        c, s_tab, a, st, br, l.r, l.p, prm, &odd);
+                    if (l.p == 0 && !last) *sp = ns;
                 }
+                if (odd) s_flag[fl] = 1;
             }
         }
         __syncthreads();
+        const bool clean = s_flag[fl] == 0;
+        if (!done && (prm.early_stop || last)) {
+            ok = clean;
+            if (prm.early_stop && clean) done = true;
+        }
+        if (last) done = true;
+        // Every lane has read s_flag before it is cleared for the next check.
+        if (!__syncthreads_or(!done)) break;
+        if (lane_ok && ug == 0) s_flag[fl] = 0;
         // Variable phase: APP = L + the new R of every check, ascending check.
         if (!done) {
             for (int v = ug; v < N; v += lanes) {

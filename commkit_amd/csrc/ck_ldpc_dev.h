@@ -680,9 +680,11 @@ constexpr int kSlotStep = 2;
 //     bit-reversed into edge order once per row;
 //   - R_e's sign bit is the row parity's flipped by Q_e's sign bit: the
 //     magnitude with the parity's sign bit, XORed with Q_e & 0x80000000.
-// PAR (with UPDATE_APP): *hard_parity = the parity of the row's hard
-// decisions on the updated APP (the XOR of their sign bits), the row's
-// syndrome bit once no later row of the iteration touches its bits.
+// PAR: *hard_parity = the parity of the row's hard decisions (the XOR of
+// the APPs' sign bits): with UPDATE_APP on the updated APP, the row's
+// syndrome bit once no later row of the iteration touches its bits; without
+// (flooding) on the APP the row reads, the syndrome bit of the previous
+// iteration's result.
 template <int P, int NT, bool UPDATE_APP, bool PAR = false>
 __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, const CheckState st,
                                                   int s0, int dc, int r, int p, int Z,
@@ -694,10 +696,15 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
     float* ar = a + r;
     const int z4 = neg_col_bytes(Z);
     const unsigned rr = row_rank(r);
+    unsigned hard = 0u;  // PAR: XOR of the read (flooding) or updated APPs' bit patterns
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         addr[t] = &lds_at(ar, qc_off(tab[s0 + t * P + p], rr, z4));
         q[t] = *addr[t];
+        if (PAR && !UPDATE_APP) {
+            const bool valid = (t < NT - S) || (t * P + p < dc);
+            hard ^= valid ? __float_as_uint(q[t]) : 0u;
+        }
     }
     // Pass 1: Q, min1 and min2 (min2 = med3(min1, min2, |Q|) as min1 <= min2),
     // the sign bits of Q.
@@ -754,7 +761,6 @@ __device__ __forceinline__ CheckState qc_ms_row_n(const int2* tab, float* a, con
     // (mp ^ (Q & 0x80000000)) instead of the parity folded in per edge.
     asm volatile("" : "+v"(m1p), "+v"(m2p));
     int idx = -1;
-    unsigned hard = 0u;  // PAR: XOR of the updated APPs' bit patterns
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
@@ -878,10 +884,13 @@ __device__ __forceinline__ float phi2_fast(float y) {
 // message is that product with Q's sign bit XORed in (one bitop3).  The
 // messages of the lane's edges sit P Z floats apart from its first edge's
 // slot s0 + p; a padding slot (e >= dc, possibly past the last slot) reads
-// slot s0 instead.
-template <int P, int NT, bool UPDATE_APP>
+// slot s0 instead.  PAR (one lane per row, flooding): *hard_parity = the
+// parity of the hard decisions on the APPs the row reads.
+template <int P, int NT, bool UPDATE_APP, bool PAR = false>
 __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __restrict__ Rm,
-                                            int s0, int dc, int r, int p, int Z, bool first) {
+                                            int s0, int dc, int r, int p, int Z, bool first,
+                                            unsigned* hard_parity = nullptr) {
+    static_assert(!PAR || (P == 1 && !UPDATE_APP), "read-APP parity: one lane per row, flooding");
     constexpr int S = kSlotStep<P>;
     float q[NT];
     float ph[NT];
@@ -900,12 +909,15 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
         rold[t] = first ? 0.0f : rm[valid ? t * step : -p * Z];  // padding: slot s0
     }
     float total = 0.0f;
-    unsigned neg = 0u;  // bit 31: parity of the valid Q's signs
+    unsigned neg = 0u;   // bit 31: parity of the valid Q's signs
+    unsigned hard = 0u;  // PAR: bit 31: parity of the valid APPs' signs
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int e = t * P + p;
         const bool valid = (t < NT - S) || (e < dc);
-        q[t] = *addr[t] - rold[t];
+        const float av = *addr[t];
+        if (PAR) hard ^= valid ? __float_as_uint(av) : 0u;
+        q[t] = av - rold[t];
         ph[t] = valid ? phi2_fast(fabsf(q[t]) * kLog2eF) : 0.0f;
         neg ^= valid ? __float_as_uint(q[t]) : 0u;
     }
@@ -949,33 +961,37 @@ __device__ __forceinline__ void qc_sp_row_n(const int2* tab, float* a, float* __
             rm[t * step] = rn;
         }
     }
+    if (PAR) *hard_parity = hard >> 31;  // no APP is -0.0: sign bit = (APP < 0)
 }
 
-template <int P, bool UPDATE_APP, int MAXT, int G = 1>
+template <int P, bool UPDATE_APP, int MAXT, int G = 1, bool PAR = false>
 __device__ __forceinline__ void qc_sp_dispatch(int groups, const int2* tab, float* a,
                                                float* __restrict__ Rm, int s0, int dc, int r, int p,
-                                               int Z, bool first) {
+                                               int Z, bool first, unsigned* hard_parity) {
     constexpr int S = kSlotStep<P>;
     if constexpr (S * G >= MAXT) {
-        qc_sp_row_n<P, S * G, UPDATE_APP>(tab, a, Rm, s0, dc, r, p, Z, first);
+        qc_sp_row_n<P, S * G, UPDATE_APP, PAR>(tab, a, Rm, s0, dc, r, p, Z, first, hard_parity);
     } else {
         if (groups <= G) {
-            qc_sp_row_n<P, S * G, UPDATE_APP>(tab, a, Rm, s0, dc, r, p, Z, first);
+            qc_sp_row_n<P, S * G, UPDATE_APP, PAR>(tab, a, Rm, s0, dc, r, p, Z, first,
+                                                   hard_parity);
             return;
         }
-        qc_sp_dispatch<P, UPDATE_APP, MAXT, G + 1>(groups, tab, a, Rm, s0, dc, r, p, Z, first);
+        qc_sp_dispatch<P, UPDATE_APP, MAXT, G + 1, PAR>(groups, tab, a, Rm, s0, dc, r, p, Z, first,
+                                                        hard_parity);
     }
 }
 
 // One sum-product check row `r` of block row `br`.
-template <int P, bool UPDATE_APP, int MAXT = kMaxDegree / P>
+template <int P, bool UPDATE_APP, int MAXT = kMaxDegree / P, bool PAR = false>
 __device__ __forceinline__ void qc_sp_row(const ck_ldpc_code& c, const int2* tab, float* a,
-                                          float* __restrict__ Rm, int br, int r, int p, bool first) {
+                                          float* __restrict__ Rm, int br, int r, int p, bool first,
+                                          unsigned* hard_parity = nullptr) {
     const int s0 = ld_uniform(c.qc_row_ptr, br);
     const int dc = ld_uniform(c.qc_row_ptr, br + 1) - s0;
     const int groups = ((dc + P - 1) / P + kSlotStep<P> - 1) / kSlotStep<P>;
-    qc_sp_dispatch<P, UPDATE_APP, MAXT>(groups, tab, a, Rm, s0, dc, r, p, c.lifting,
-                                                  first);
+    qc_sp_dispatch<P, UPDATE_APP, MAXT, 1, PAR>(groups, tab, a, Rm, s0, dc, r, p, c.lifting,
+                                                first, hard_parity);
 }
 
 }  // namespace ckdev

@@ -127,6 +127,27 @@ def test_sum_product_matches(gpu, name, schedule):
         print(f"sum-product {name} {schedule} {tiles}: divergent frames {diverged.tolist()}")
 
 
+@pytest.mark.parametrize("name,lo,hi,rows", [("r1/2-z27", 1.9, 2.6, 16), ("r4/5-z360", 3.0, 3.8, 8)])
+def test_sum_product_flooding_early_stop(gpu, name, lo, hi, rows):
+    # The flooding kernel's stop rule is the parity of the hard decisions its
+    # check phase reads.  Frames with mean LLRs across each code's waterfall
+    # converge after 2 to 16 iterations or not at all (the oracle runs the
+    # first four r1/2-z27 frames and the first r4/5-z360 frame to the limit);
+    # each must stop where the oracle stops, with its bits and valid flag.
+    code = CODES[name][0]()
+    r = np.random.default_rng(14)
+    mean = np.linspace(lo, hi, rows)[:, None]
+    llrs = (mean + 1.5 * r.standard_normal((rows, code.n))).astype(np.float32)
+    method = SumProduct(max_iterations=25, early_stop=True, schedule="flooding")
+    want = oracle_decode(code, llrs, method)
+    got = gpu_decode(code, llrs, method)
+    assert len(np.unique(want[3])) > 1, "the frames should stop at different iterations"
+    np.testing.assert_array_equal(got[3], want[3])
+    np.testing.assert_array_equal(got[2], want[2])
+    np.testing.assert_array_equal(got[0], want[0])
+    np.testing.assert_allclose(got[1], want[1], rtol=1e-3, atol=1e-3)
+
+
 def test_noiseless_and_random(gpu):
     code = seeded_code("r2/3-z96")
     u = np.random.default_rng(5).integers(0, 2, (5, code.k))
