@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, int G,
     int small, Params prm) {
     extern __shared__ __attribute__((aligned(16))) float lds[];  // 16-byte check states
-    __shared__ int s_flag[kMaxFrames];
+    __shared__ int s_flag[2][kMaxFrames];
     __shared__ int2 s_tab[kTabEntries];
     const int N = c.num_variables;
     const int Z = c.lifting;
@@ -245,9 +245,12 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
         L = llr + fr * c.n;
     }
 
-    // s_flag[f]: some row of frame f has odd hard-decision parity on the APP
-    // its check phase read.
-    if (lane_ok && ug == 0) s_flag[fl] = 0;
+    // s_flag[it & 1][f]: some row of frame f has odd hard-decision parity on
+    // the APP the check phase of iteration `it` read.  Two buffers: one is
+    // cleared for the next iteration while the other is read, so no barrier
+    // separates the reads from the clearing.
+    for (int i = threadIdx.x; i < 2 * kMaxFrames; i += blockDim.x)
+        s_flag[i / kMaxFrames][i % kMaxFrames] = 0;
     __syncthreads();
     bool done = !l.live;
     bool ok = false;
@@ -256,6 +259,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     for (;;) {
         const bool last = it >= prm.max_iterations;
         const bool first = it == 0;
+        int* flag = s_flag[it & 1];
         // Check phase: every row reads the previous iteration's APP, so the
         // parities of its hard decisions make the syndrome of that APP, the
         // check before iteration `it` (the APP after `it` iterations).  At
@@ -274,19 +278,25 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
                         c, s_tab, a, st, br, l.r, l.p, prm, &odd);
                     if (l.p == 0 && !last) *sp = ns;
                 }
-                if (odd) s_flag[fl] = 1;
+                if (odd) flag[fl] = 1;
             }
         }
         __syncthreads();
-        const bool clean = s_flag[fl] == 0;
+        const bool clean = flag[fl] == 0;
         if (!done && (prm.early_stop || last)) {
             ok = clean;
             if (prm.early_stop && clean) done = true;
         }
         if (last) done = true;
-        // Every lane has read s_flag before it is cleared for the next check.
-        if (!__syncthreads_or(!done)) break;
-        if (lane_ok && ug == 0) s_flag[fl] = 0;
+        // Whether any frame of the workgroup runs on, the same on every lane:
+        // a frame that has stopped, or holds no frame, raised no flag.
+        bool more = !last;
+        if (more && prm.early_stop) {
+            more = false;
+            for (int f = 0; f < F; ++f) more = more || flag[f] != 0;
+        }
+        if (!more) break;
+        if (lane_ok && ug == 0) s_flag[(it + 1) & 1][fl] = 0;  // read last iteration
         // Variable phase: APP = L + the new R of every check, ascending check.
         if (!done) {
             for (int v = ug; v < N; v += lanes) {
