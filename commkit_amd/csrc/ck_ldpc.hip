@@ -180,6 +180,40 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
 // syndrome check takes no pass of its own: the check phase reads every
 // row's APPs, and the parity of their signs is the row's syndrome bit for
 // the APP the previous iteration left (the one the stop decision is about).
+// A variable's new APP in the flooding variable phase: l0 plus the messages
+// of its checks (column edges t0 .. t1 - 1) in ascending check order.  B
+// edges at a time: each edge's chain of table and message reads is issued
+// before the first of their sums (with the tables and messages in HBM the
+// chains overlap; from LDS, B = 1 measures faster).
+constexpr int kVarBatch = 4;
+
+template <bool SP, int B>
+__device__ __forceinline__ float var_sum(float sum, int t0, int t1, int cc, int Z,
+                                         const int* col_slot, const int* slot_shift,
+                                         const int* slot_row, const int* slot_pos,
+                                         const CheckState* st_base, const float* Rm) {
+    for (int tb = t0; tb < t1; tb += B) {
+        float rv[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int t = tb + u < t1 ? tb + u : t0;  // in the column
+            const int s = col_slot[t];
+            int rr = cc - slot_shift[s];
+            rr += (rr < 0) ? Z : 0;
+            if (SP) {
+                rv[u] = Rm[static_cast<size_t>(s) * Z + rr];
+            } else {
+                const CheckState st = st_base[static_cast<size_t>(slot_row[s]) * Z + rr];
+                rv[u] = state_msg(st, slot_pos[s]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (tb + u < t1) sum = sum + rv[u];
+    }
+    return sum;
+}
+
 template <bool SP, int P>
 __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
@@ -303,20 +337,14 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
                 const int j = v / Z;
                 const int cc = v - j * Z;
                 // -0.0 as +0.0, as at the load (the LDS copy is the loaded APP)
-                float sum = small ? L[v] : (v < c.n ? L[v] + 0.0f : 0.0f);
-                for (int t = col_ptr[j]; t < col_ptr[j + 1]; ++t) {
-                    const int s = col_slot[t];
-                    int rr = cc - slot_shift[s];
-                    rr += (rr < 0) ? Z : 0;
-                    float rv;
-                    if (SP) {
-                        rv = Rm[static_cast<size_t>(s) * Z + rr];
-                    } else {
-                        const CheckState st = st_base[static_cast<size_t>(slot_row[s]) * Z + rr];
-                        rv = state_msg(st, slot_pos[s]);
-                    }
-                    sum = sum + rv;
-                }
+                const float l0 = small ? L[v] : (v < c.n ? L[v] + 0.0f : 0.0f);
+                const int t0 = col_ptr[j];
+                const int t1 = col_ptr[j + 1];
+                const float sum =
+                    small ? var_sum<SP, 1>(l0, t0, t1, cc, Z, col_slot, slot_shift, slot_row,
+                                           slot_pos, st_base, Rm)
+                          : var_sum<SP, kVarBatch>(l0, t0, t1, cc, Z, col_slot, slot_shift,
+                                                   slot_row, slot_pos, st_base, Rm);
                 a[v] = sum;
             }
         }
