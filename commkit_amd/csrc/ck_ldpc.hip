@@ -74,7 +74,10 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
     SymbolSource sym) {
     // 16-byte aligned: float4 frame loads, the fused demapping's tables
     extern __shared__ __attribute__((aligned(16))) float lds[];
-    __shared__ int s_flag[kMaxFrames];
+    // s_flag[it & 1]: the syndrome's flags of the check after iteration
+    // `it` (min-sum: the last block row's parities, set during the
+    // iteration); the other buffer is cleared meanwhile.
+    __shared__ int s_flag[2][kMaxFrames];
     __shared__ int2 s_tab[kTabEntries];
     const int N = c.num_variables;
     const int Z = c.lifting;
@@ -119,20 +122,32 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
         // last block row's hard-decision parities (final: no later row of
         // the iteration touches its bits) have already settled s_flag.
         const bool last = it >= prm.max_iterations;
-        if (prm.early_stop || last) {
-            const bool now = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, !done, F, s_flag,
-                                            !SP && it > 0 && mb > 1);
-            if (!done) {
-                ok = now;
-                if (prm.early_stop) done = now;
+        int* flag = s_flag[(it + 1) & 1];  // set by the last iteration's last block row
+        // One frame whose last block row left an odd parity: not satisfied,
+        // and the workgroup runs on; every lane reads the same flag, so the
+        // syndrome's pass and the workgroup's OR are skipped alike.
+        const bool rejected = F == 1 && !SP && mb > 1 && it > 0 && prm.early_stop && !last &&
+                              flag[0] == 1;
+        if (rejected) {
+            ok = false;
+        } else {
+            if (prm.early_stop || last) {
+                const bool now = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, !done, F, flag,
+                                                !SP && it > 0 && mb > 1);
+                if (!done) {
+                    ok = now;
+                    if (prm.early_stop) done = now;
+                }
             }
+            if (last || !__syncthreads_or(!done)) break;
         }
-        if (last || !__syncthreads_or(!done)) break;
         const bool first = it == 0;
-        // Cleared for the parities of this iteration's last block row (the
-        // barrier above ordered the syndrome's reads before this write, the
-        // barriers after the block rows order it before theirs).
-        if (!SP && mb > 1 && lane_ok && l.u == 0) s_flag[l.f] = 0;
+        // Cleared for the parities of this iteration's last block row: last
+        // read before the previous iteration's block rows, whose barriers
+        // order those reads before this write and this write before the
+        // last block row's.
+        int* next = s_flag[it & 1];
+        if (!SP && mb > 1 && lane_ok && l.u == 0) next[l.f] = 0;
         for (int br = 0; br < mb; ++br) {
             if (!done) {
                 if (SP) {
@@ -153,7 +168,7 @@ __attribute__((amdgpu_waves_per_eu(SHORT == 2 ? 4 : (SHORT == 1 ? 3 : 1)))) void
                         unsigned odd = 0u;
                         ns = qc_ms_row<P, true, kRowSlots<P, SHORT>, true>(c, s_tab, a, st, br,
                                                                            l.r, l.p, prm, &odd);
-                        if (odd) s_flag[l.f] = 1;
+                        if (odd) next[l.f] = 1;
                     }
                     if (l.p == 0) st_base[static_cast<size_t>(br) * Z + l.r] = ns;
                     if (mb == 1) st_next = ns;
