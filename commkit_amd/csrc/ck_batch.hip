@@ -99,7 +99,7 @@ __device__ __forceinline__ void refill(const BatchClass& bc, unsigned* counter, 
         if (prm.early_stop) {
             const bool ok = qc_syndrome_ok(c, tab, lds, bits, l, lane_ok, fresh, F, s_flag);
             done0 = fresh && ok;
-            if (done0) store_frame(c, a, frame, l.u, Z * P, 0, true, bc.app, bc.bits, bc.iters, bc.valid);
+            if (done0) store_frame<false>(c, a, frame, l.u, Z * P, 0, true, bc.app, bc.bits, bc.iters, bc.valid);
         }
         if (fresh && !done0) {
             active = true;
@@ -167,7 +167,7 @@ __device__ __forceinline__ void run_class(BatchDesc* __restrict__ d, int ci, flo
         const bool ok = qc_syndrome_ok(c, s_tab, lds, bits, l, lane_ok, check, F, s_flag);
         const bool fin = active && ((prm.early_stop && ok) || my_it >= prm.max_iterations);
         if (fin) {
-            store_frame(c, a, frame, l.u, Z * P, my_it, ok, bc.app, bc.bits, bc.iters, bc.valid);
+            store_frame<false>(c, a, frame, l.u, Z * P, my_it, ok, bc.app, bc.bits, bc.iters, bc.valid);
             active = false;
             if (l.u == 0) s_need[l.f] = 1;
         }

@@ -284,12 +284,65 @@ __device__ inline void load_frames_demap(float* lds, const SymbolSource& src, vo
 
 // APP, hard info bits, iterations and valid of one frame, written by the
 // `lanes` lanes numbered `u` = 0..lanes-1 of that frame.
+template <bool VEC = true>
 __device__ inline void store_frame(const ck_ldpc_code& c, const float* a, long long frame, int u,
                             int lanes, int it, bool ok, float* __restrict__ app_out,
                             signed char* __restrict__ bits_out, int* __restrict__ iters_out,
                             unsigned char* __restrict__ valid_out) {
-    // kLoadBatch LDS reads of a lane in flight before its stores.
+    // kLoadBatch LDS reads of a lane in flight before its stores.  VEC, with
+    // n and k multiples of 4, 16-byte aligned rows and the information bits
+    // first (no info table): the APP moves as 16-byte vectors and the hard
+    // bits as 4-byte words of four sign bytes (the batched kernel, short of
+    // registers, keeps the scalar form).
     float* dst = app_out + frame * c.n;
+    const bool vec = VEC && ((c.n & 3) == 0) && ((c.k & 3) == 0) && c.info == nullptr &&
+                     ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) &&
+                     ((reinterpret_cast<uintptr_t>(a) & 15) == 0) &&
+                     (bits_out == nullptr || ((reinterpret_cast<uintptr_t>(bits_out + frame * c.k) & 3) == 0));
+    if (vec) {
+        const float4* a4 = reinterpret_cast<const float4*>(a);
+        float4* d4 = reinterpret_cast<float4*>(dst);
+        const int n4 = c.n >> 2;
+        for (int v0 = u; v0 < n4; v0 += kLoadBatch * lanes) {
+            float4 x[kLoadBatch];
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int v = v0 + k * lanes;
+                x[k] = v < n4 ? a4[v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int v = v0 + k * lanes;
+                if (v < n4) d4[v] = x[k];
+            }
+        }
+        if (bits_out != nullptr) {
+            unsigned* b4 = reinterpret_cast<unsigned*>(bits_out + frame * c.k);
+            const int k4 = c.k >> 2;
+            for (int i0 = u; i0 < k4; i0 += kLoadBatch * lanes) {
+                float4 x[kLoadBatch];
+#pragma unroll
+                for (int k = 0; k < kLoadBatch; ++k) {
+                    const int i = i0 + k * lanes;
+                    x[k] = a4[i < k4 ? i : 0];
+                }
+#pragma unroll
+                for (int k = 0; k < kLoadBatch; ++k) {
+                    const int i = i0 + k * lanes;
+                    // byte j = hard decision of bit 4i + j (APP < 0), little-endian
+                    const unsigned w = (x[k].x < 0.0f ? 1u : 0u) | (x[k].y < 0.0f ? 0x100u : 0u) |
+                                       (x[k].z < 0.0f ? 0x10000u : 0u) |
+                                       (x[k].w < 0.0f ? 0x1000000u : 0u);
+                    if (i < k4) b4[i] = w;
+                }
+            }
+        }
+        if (u == 0) {
+            iters_out[frame] = it;
+            valid_out[frame] = ok ? 1 : 0;
+        }
+        return;
+    }
     for (int v0 = u; v0 < c.n; v0 += kLoadBatch * lanes) {
         float x[kLoadBatch];
 #pragma unroll
