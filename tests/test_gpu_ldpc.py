@@ -199,6 +199,21 @@ def test_punctured_bits_are_erasures(gpu):
     assert res.valid.all() and res.llrs.shape == (3 * code.n,)
 
 
+@pytest.mark.parametrize("schedule", ["layered", "flooding"])
+def test_odd_length_code_bit_exact(gpu, schedule):
+    # n = 35 (the small QC code with its last 5 variables punctured): the
+    # frame store takes its one-float form instead of 16-byte vectors, and
+    # the loads their scalar form; every kernel and tile still equals the
+    # oracle bit for bit.
+    code = quiet(LDPCCode.quasi_cyclic, small_qc().__dict__["_qc"][0], lifting=5, punctured=5)
+    assert code.n % 4 == 3
+    llrs = _llrs(code, 9, 1.6, seed=15)
+    method = MinSum(scale=0.75, max_iterations=12, schedule=schedule)
+    want = oracle_decode(code, llrs, method)
+    for tiles in (None, (1, 0, 0, -1), (2, 1, 1, 0), (2, 2, 2, 0), (2, 3, 4, 0)):
+        assert_identical(gpu_decode(code, llrs, method, tiles), want, f"{schedule} {tiles}")
+
+
 def test_decode_contract(gpu):
     import torch
 
