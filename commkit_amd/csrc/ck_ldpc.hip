@@ -38,7 +38,9 @@
 //               consecutive words.
 //   qc_flooding the flooding schedule on the same layout: every check uses
 //               the previous iteration's APP; then APP = L + sum of R over
-//               the variable's checks, in ascending check order.
+//               the variable's checks, in ascending check order.  The
+//               check phase's reads also give the syndrome of the APP it
+//               reads, so the stop rule needs no pass of its own.
 //   generic     any code through its CSR schedule, one workgroup per frame,
 //               rows of a layer strided over the lanes, APP in LDS when it
 //               fits (else in the workspace), R in the workspace in CSR edge
