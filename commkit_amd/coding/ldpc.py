@@ -770,8 +770,8 @@ def _decode_symbols(code: LDPCCode, x: Any, constellation: Any, noise_var: Any,
     bits a symbol), ``noise_var`` a float or ``(R,)`` per frame; the same
     ``(bits, app, valid, iterations)`` as ``_decode`` of the demapper's
     LLRs, bit for bit (``ck_ldpc_decode_symbols``: the decoder's frame load
-    demaps, and the LLRs never pass through HBM).  Quasi-cyclic codes on
-    the layered kernel."""
+    demaps, and the LLRs never pass through HBM).  Quasi-cyclic codes, on
+    the layered or the flooding kernel."""
     import torch
 
     from ..mapping.demap import axis_tables, closed_form

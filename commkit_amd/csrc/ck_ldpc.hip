@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     ck_ldpc_code c, const float* __restrict__ llr, float* __restrict__ app_out,
     signed char* __restrict__ bits_out, int* __restrict__ iters_out,
     unsigned char* __restrict__ valid_out, void* __restrict__ ws, long long R, int F, int G,
-    int small, Params prm) {
+    int small, Params prm, SymbolSource sym) {
     extern __shared__ __attribute__((aligned(16))) float lds[];  // 16-byte check states
     __shared__ int s_flag[2][kMaxFrames];
     __shared__ int2 s_tab[kTabEntries];
@@ -252,7 +252,12 @@ __global__ __launch_bounds__(kMaxQcThreads) void ldpc_qc_flooding(
     const int ug = g * Z * P + l.u;  // this lane among them
     float* a = lds + static_cast<size_t>(l.f < F ? l.f : 0) * N;
     stage_slot_table(c, s_tab);
-    load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
+    if (sym.x != nullptr) {  // demapping fused into the load; tables over the unused bitmap area
+        load_frames_demap(lds, sym, lds + qc_lds(F, N, Z, c.num_checks, false).bits,
+                          static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
+    } else {
+        load_frames(lds, llr, static_cast<long long>(blockIdx.x) * F, F, R, c.n, N);
+    }
     __syncthreads();
 
     const bool lane_ok = l.f < F;
@@ -920,16 +925,19 @@ void launch_qc_layered(const Plan& p, const ck_ldpc_code& c, const float* llr, f
 template <bool SP, int P>
 void launch_qc_flooding(const Plan& p, const ck_ldpc_code& c, const float* llr, float* app,
                         signed char* bits, int* iters, unsigned char* valid, void* ws,
-                        long long R, const Params& prm, hipStream_t s) {
+                        long long R, const Params& prm, hipStream_t s, const SymbolSource& sym) {
     auto kernel = ldpc_qc_flooding<SP, P>;
     const long long groups = (R + p.F - 1) / p.F;
-    const size_t lds = p.small ? flood_lds(p.F, c.num_variables, c.lifting, c.num_checks,
-                                           c.num_edges, c.num_variables / c.lifting,
-                                           c.num_slots, SP).end * 4
+    // p.lds_bytes covers the QC layout, and with fused demapping its tables.
+    const size_t lds = p.small ? std::max<size_t>(flood_lds(p.F, c.num_variables, c.lifting,
+                                                            c.num_checks, c.num_edges,
+                                                            c.num_variables / c.lifting,
+                                                            c.num_slots, SP).end * 4,
+                                                  p.lds_bytes)
                                : p.lds_bytes;
     allow_lds(kernel, lds);
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(groups)), dim3(p.threads), lds, s, c,
-                       llr, app, bits, iters, valid, ws, R, p.F, p.G, p.small ? 1 : 0, prm);
+                       llr, app, bits, iters, valid, ws, R, p.F, p.G, p.small ? 1 : 0, prm, sym);
 }
 
 template <bool SP, bool FLOOD, bool APP_LDS, int MAXD>
@@ -968,7 +976,7 @@ extern "C" int64_t ck_ldpc_workspace_bytes(const ck_ldpc_code* code, const ck_ld
 }
 
 // The decode of ck_ldpc_decode; with sym.x set (ck_ldpc_decode_symbols) the
-// frames come as symbols, demapped in the quasi-cyclic layered kernel's load.
+// frames come as symbols, demapped in the quasi-cyclic kernel's load.
 static int32_t decode_impl(const ck_ldpc_code* code, const ck_ldpc_method* method,
                            const float* llrs, const SymbolSource& sym, float* app, int8_t* bits,
                            int32_t* iterations, uint8_t* valid, int64_t R, void* workspace,
@@ -979,8 +987,14 @@ static int32_t decode_impl(const ck_ldpc_code* code, const ck_ldpc_method* metho
     if (R == 0) return CK_OK;
     CK_REQUIRE(R <= 0x7fffffffLL, "R = %lld exceeds one launch", static_cast<long long>(R));
     CK_REQUIRE((llrs || sym.x) && app && iterations && valid, "NULL input or output");
-    CK_REQUIRE(sym.x == nullptr || (p.kind == 2 && !p.flood),
-               "demapping fused into the decode needs the quasi-cyclic layered kernel");
+    CK_REQUIRE(sym.x == nullptr || p.kind == 2,
+               "demapping fused into the decode needs a quasi-cyclic kernel (layered or flooding)");
+    // The flooding kernel's HBM form reads the channel LLRs from the input
+    // in every variable phase; with symbols in there is no such buffer, so
+    // only its LDS form (channel LLRs copied to LDS after the load) fuses.
+    CK_REQUIRE(sym.x == nullptr || !p.flood || p.small,
+               "fused demapping with the flooding schedule needs a code whose channel LLRs, "
+               "messages and column tables fit in LDS beside its APPs");
     if (sym.x != nullptr) {
         // The demapping tables go past the frames' APPs, over the sign
         // bitmaps and check states (unused until the load ends).
@@ -1011,10 +1025,10 @@ static int32_t decode_impl(const ck_ldpc_code* code, const ck_ldpc_method* metho
     unsigned char* v = valid;
     if (p.kind == 2) {
         if (p.flood) {
-            if (p.sp) launch_qc_flooding<true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 1) launch_qc_flooding<false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else if (p.P == 2) launch_qc_flooding<false, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
-            else launch_qc_flooding<false, 4>(p, c, llrs, app, b, it, v, workspace, R, prm, s);
+            if (p.sp) launch_qc_flooding<true, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 1) launch_qc_flooding<false, 1>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else if (p.P == 2) launch_qc_flooding<false, 2>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
+            else launch_qc_flooding<false, 4>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
         } else if (p.sp) {
             if (p.P == 1) launch_qc_layered<true, 1, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);
             else if (p.P == 2) launch_qc_layered<true, 2, false>(p, c, llrs, app, b, it, v, workspace, R, prm, s, sym);

@@ -244,8 +244,9 @@ int32_t ck_ldpc_decode(const ck_ldpc_code* code, const ck_ldpc_method* method,
  * inv_rows is not NULL.  Outputs, workspace and results are those of
  * ck_ldpc_decode of the LLRs ck_llr_separable_maxlog (or
  * ck_llr_separable_rows with closed = 1) writes for the same symbols, bit
- * for bit.  Quasi-cyclic codes with the layered schedule only (CK_EINVAL
- * otherwise).  No counterpart in the reference, whose receive tail runs
+ * for bit.  Quasi-cyclic codes, layered schedule, or flooding when the
+ * code's channel LLRs, messages and column tables fit in LDS beside its
+ * APPs (CK_EINVAL otherwise, and for the generic kernel).  No counterpart in the reference, whose receive tail runs
  * compute_llr (commkit/mapping/llr.py:24-95) and then the decoder
  * (commkit/coding/ldpc.py:566-587) as two launches. */
 int32_t ck_ldpc_decode_symbols(const ck_ldpc_code* code, const ck_ldpc_method* method,

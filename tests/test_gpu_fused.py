@@ -4,8 +4,9 @@
 tail gives: the closed-form max-log LLRs of the same symbols
 (``compute_llr`` / ``llr_rows``), decoded by ``ck_ldpc_decode`` -- APP LLRs
 bit for bit (signed zeros included), hard bits, iteration counts and valid
-flags, for every tile of the quasi-cyclic layered kernel, min-sum and
-sum-product, QPSK to 256-QAM, one noise variance or one per frame.  The
+flags, for every tile of the quasi-cyclic layered and flooding kernels,
+min-sum and sum-product, QPSK to 256-QAM, one noise variance or one per
+frame.  The
 two-launch tail itself is pinned to the oracle by the decoder and demapper
 tests.
 """
@@ -153,10 +154,11 @@ def test_fused_rejects(gpu):
     code = seeded_code("r1/2-z27")
     c = Constellation.qam(64)
     x = torch.zeros((2, code.n // 6), dtype=torch.complex64, device="cuda")
-    with pytest.raises(RuntimeError, match="quasi-cyclic layered"):
+    with pytest.raises(RuntimeError, match="quasi-cyclic kernel"):
         ldpc_module._decode_symbols(code, x, c, 0.1, MinSum(), tiles=(1, 0, 0, -1))
-    with pytest.raises(RuntimeError, match="quasi-cyclic layered"):
-        ldpc_module._decode_symbols(code, x, c, 0.1, MinSum(schedule="flooding"))
+    with pytest.raises(RuntimeError, match="quasi-cyclic kernel"):
+        ldpc_module._decode_symbols(code, x, c, 0.1, MinSum(schedule="flooding"),
+                                    tiles=(1, 0, 0, -1))
     with pytest.raises(ValueError):
         ldpc_module._decode_symbols(code, x, Constellation.psk(8), 0.1, MinSum())
     with pytest.raises(ValueError):
@@ -169,3 +171,60 @@ def test_fused_rejects(gpu):
         ldpc_module._decode_symbols(big, xb, Constellation.qam(1024), 0.1, MinSum())
     bits, app, valid, iters = ldpc_module._decode_symbols(code, x[:0], c, 0.1, MinSum())
     assert bits.shape == (0, code.k) and app.shape == (0, code.n)
+
+
+# Flooding tiles: (kernel, frames per group, lanes per row, state placement).
+FLOOD_TILES = [(2, 0, 0, -1), (2, 1, 1, 0), (2, 1, 2, 0), (2, 2, 4, 0), (2, 3, 1, 0)]
+
+
+@pytest.mark.parametrize("name,order,rows,snr_db", [("r1/2-z27", 16, 64, 8.0),
+                                                    ("r1/2-z27", 256, 7, 16.0),
+                                                    ("r2/3-z96", 64, 9, 13.0)])
+def test_fused_flooding(gpu, name, order, rows, snr_db):
+    # These codes run the flooding kernel's LDS form (the tables are staged
+    # over the area its channel LLRs and states take after the load).
+    import torch
+
+    code = seeded_code(name)
+    c = Constellation.qam(order)
+    x_np, nv = _symbols(c, rows, code.n // c.bits_per_symbol, snr_db, seed=order + rows + 1)
+    x = torch.from_numpy(x_np).cuda()
+    for method in (MinSum(scale=0.75, max_iterations=20, schedule="flooding"),
+                   SumProduct(max_iterations=8, schedule="flooding")):
+        tiles_list = FLOOD_TILES if isinstance(method, MinSum) else [(2, 0, 0, -1), (2, 1, 1, 0)]
+        for tiles in tiles_list:
+            try:
+                want = _two_launch(code, c, x, nv, method, tiles)
+            except RuntimeError:  # the tile does not fit this code: refused alike
+                with pytest.raises(RuntimeError):
+                    _fused(code, c, x, nv, method, tiles)
+                continue
+            _assert_same(_fused(code, c, x, nv, method, tiles), want,
+                         f"{name} {order} {type(method).__name__} {tiles}")
+
+
+def test_fused_flooding_refuses_the_hbm_form(gpu):
+    # r4/5-z360 (n = 18000) floods in the HBM form, which reads the channel
+    # LLRs from the input every iteration: with symbols in, the call is
+    # refused (checked on the host, before any launch).
+    import torch
+
+    code = seeded_code("r4/5-z360")
+    x = torch.zeros((2, code.n // 6), dtype=torch.complex64, device="cuda")
+    with pytest.raises(RuntimeError, match="fit in LDS"):
+        ldpc_module._decode_symbols(code, x, Constellation.qam(64), 0.1,
+                                    MinSum(schedule="flooding"))  # fmt: skip
+
+
+def test_fused_flooding_noise_per_frame(gpu):
+    import torch
+
+    code = seeded_code("r1/2-z27")
+    c = Constellation.qam(16)
+    rows = 11
+    x_np, _ = _symbols(c, rows, code.n // 4, 8.0, seed=21)
+    x = torch.from_numpy(x_np).cuda()
+    nv = 10 ** (-np.linspace(6.0, 12.0, rows) / 10)
+    method = MinSum(scale=0.75, max_iterations=20, schedule="flooding")
+    _assert_same(_fused(code, c, x, nv, method, None), _two_launch(code, c, x, nv, method, None),
+                 "flooding per-frame")  # fmt: skip
